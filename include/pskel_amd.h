@@ -1,0 +1,206 @@
+/*
+ * pskel_amd.h -- C ABI of the MI355X (gfx950) iterative-stencil library.
+ *
+ * This is the boundary the PSkel drop-in headers (PSkel.h and friends in this
+ * directory) call for every GPU method.  Plain C types only: pointers, sizes,
+ * fixed-layout structs, status codes.  No exceptions cross it and nothing in it
+ * calls exit() -- the reference's gpuErrchk prints and exits
+ * (include/PSkelDefs.h:50-56); here every failure is a status plus a message
+ * (pskel_last_error).
+ *
+ * Reference paths below are relative to the PSkel source tree.
+ */
+#ifndef PSKEL_AMD_H
+#define PSKEL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSKEL_AMD_ABI_VERSION 1
+#define PSKEL_MAX_POINTS 64
+
+typedef enum pskel_status {
+    PSKEL_OK = 0,
+    PSKEL_ERR_INVALID_ARGUMENT = 1,
+    PSKEL_ERR_OUT_OF_MEMORY = 2,
+    PSKEL_ERR_NO_DEVICE = 3,
+    PSKEL_ERR_LAUNCH = 4,
+    PSKEL_ERR_UNSUPPORTED = 5,
+    PSKEL_ERR_RUNTIME = 6
+} pskel_status_t;
+
+/* Element types.  U8/F32 are what Array2D<unsigned char>/Array2D<float> hold;
+ * BF16/F16 are device storage formats (host side stays F32, or raw 16-bit words);
+ * F64 is accepted only as a host type for upload/download conversions. */
+typedef enum pskel_dtype {
+    PSKEL_DTYPE_U8 = 1,
+    PSKEL_DTYPE_F32 = 2,
+    PSKEL_DTYPE_F64 = 3,
+    PSKEL_DTYPE_BF16 = 4,
+    PSKEL_DTYPE_F16 = 5
+} pskel_dtype_t;
+
+/* Stencil families.  A user stencilKernel written against Mask::get in mask
+ * order (include/PSkelStencil.h:121-156, include/PSkelMask.hpp:186-199, 292-305)
+ * computes one of these:
+ *   SCALED_SUM   out = coeff * (((p0 + p1) + p2) + ...)
+ *   WEIGHTED_SUM out = ((w0*p0 + w1*p1) + w2*p2) + ...     (w_i = Mask weight i)
+ *   LIFE         n = p0 + p1 + ...; out = (n == 3 || (n == 2 && centre == 1))
+ * Each product and sum rounds on its own (no fused multiply-add), so results
+ * are bit-identical to runIterativeCPU built the same way. */
+typedef enum pskel_kind {
+    PSKEL_KIND_SCALED_SUM = 1,
+    PSKEL_KIND_WEIGHTED_SUM = 2,
+    PSKEL_KIND_LIFE = 3
+} pskel_kind_t;
+
+/* 3D boundary rule (Stencil3D visits every cell, include/PSkelStencil.hpp:2095-2113):
+ *   COPY  cells within `radius` of a face keep their value (out = in)
+ *   HALO  every cell is updated; neighbours outside read halo_value
+ *         (Mask3D::get, include/PSkelMask.hpp:186-199)
+ * 2D grids always update the interior [r, N-r) only (Stencil2D::runOpenMP,
+ * include/PSkelStencil.hpp:2235-2255); the ring keeps each buffer's values. */
+typedef enum pskel_boundary {
+    PSKEL_BOUNDARY_COPY = 0,
+    PSKEL_BOUNDARY_HALO = 1
+} pskel_boundary_t;
+
+/* Mask2D/Mask3D + the kernel family (include/PSkelMask.h:47-262).
+ * offsets: (dh, dw) per point for 2D, (dh, dw, dd) for 3D, as Mask::set takes them. */
+typedef struct pskel_stencil_desc {
+    int32_t kind;
+    int32_t npoints;
+    int32_t offsets[3 * PSKEL_MAX_POINTS];
+    float weights[PSKEL_MAX_POINTS];
+    float coeff;
+    int32_t radius;     /* 0: Mask::getRange() of the offsets (include/PSkelMask.hpp:308-334) */
+    int32_t boundary;   /* pskel_boundary_t, 3D only */
+    float halo_value;   /* 3D HALO */
+    int32_t reserved[4];
+} pskel_stencil_desc_t;
+
+typedef enum pskel_variant {
+    PSKEL_VARIANT_PLAIN = 0, /* one lane per cell, caller's block shape: the runCUDA contract */
+    PSKEL_VARIANT_TUNED = 1  /* register/LDS-tiled kernels, optional temporal tiles */
+} pskel_variant_t;
+
+/* Launch configuration (replaces the GPUBlockSizeX/Y, tilingWidth/Height and
+ * innerIterations arguments of runIterativeGPU / runIterativeTilingGPU,
+ * include/PSkelStencil.h:258-297). Zero fields take defaults. */
+typedef struct pskel_launch_cfg {
+    int32_t variant;        /* pskel_variant_t */
+    int32_t block_x;        /* plain: threads along the contiguous axis (w in 2D, d in 3D) */
+    int32_t block_y;        /* plain: threads along the next axis (h in 2D, w in 3D) */
+    int32_t tile_x;         /* tuned: cells per lane along the contiguous axis (vector width) */
+    int32_t tile_y;         /* tuned: rows per lane (2D) / w-rows per wave (3D) */
+    int32_t tile_z;         /* tuned 3D: planes per block segment along h; 2D: waves per block */
+    int32_t time_tile;      /* tuned: sweeps fused per launch (temporal depth k); 0/1 = none */
+    int32_t residual_every; /* FP64 residual ||u^k - u^(k-1)||_2 every n sweeps; 0 = never */
+    int32_t use_graph;      /* capture the launch sequence in a hipGraph and replay it once */
+    int32_t reserved[3];
+    void* stream;           /* hipStream_t; NULL: the grid's own stream */
+} pskel_launch_cfg_t;
+
+typedef struct pskel_run_stats {
+    double elapsed_ms;        /* device time between events around the launch sequence */
+    int64_t launches;         /* kernel launches issued for the sweeps */
+    int64_t sweeps;           /* sweeps performed (== iterations) */
+    double* residuals;        /* caller buffer for FP64 residuals (may be NULL) */
+    int64_t residual_capacity;
+    int64_t residual_count;   /* residuals produced (written up to capacity) */
+} pskel_run_stats_t;
+
+typedef struct pskel_grid pskel_grid_t;
+
+typedef struct pskel_grid_desc {
+    int32_t ndim;        /* 2 or 3 */
+    int32_t storage;     /* pskel_dtype_t on the device: U8, F32, BF16 or F16 */
+    size_t width;        /* Array2D/Array3D width  (include/PSkelArray.h:295-384) */
+    size_t height;       /* height */
+    size_t depth;        /* depth (1 for 2D); the contiguous axis in 3D (include/PSkelArray.hpp:642) */
+    float store_offset;  /* BF16/F16 storage holds (u - store_offset) */
+    int32_t device;      /* HIP device ordinal */
+} pskel_grid_desc_t;
+
+typedef enum pskel_buffer {
+    PSKEL_BUFFER_INPUT = 0,  /* the Stencil's `input` array */
+    PSKEL_BUFFER_OUTPUT = 1  /* the Stencil's `output` array */
+} pskel_buffer_t;
+
+typedef struct pskel_grid_info {
+    int32_t ndim, storage, elem_bytes, parity;
+    size_t width, height, depth, pitch, bytes_per_buffer;
+    void* current;   /* device pointer of the buffer holding the latest state */
+    void* previous;  /* device pointer of the other ping-pong buffer */
+    void* stream;    /* the grid's hipStream_t */
+} pskel_grid_info_t;
+
+/* ---- library ------------------------------------------------------------ */
+int32_t pskel_abi_version(void);
+/* sizeof of the ABI structs, for bindings to verify their layout:
+ * 0 stencil_desc, 1 launch_cfg, 2 run_stats, 3 grid_desc, 4 grid_info; else 0. */
+size_t pskel_abi_struct_size(int32_t which);
+const char* pskel_status_string(pskel_status_t st);
+/* Message of the last failure on the calling thread ("" if none). */
+const char* pskel_last_error(void);
+pskel_status_t pskel_device_count(int32_t* count);
+pskel_status_t pskel_device_synchronize(void);
+/* Device name and HBM size of `device` (cudaGetDeviceProperties in runGPU,
+ * include/PSkelStencil.hpp:733-737; cudaMemGetInfo in runAutoGPU, :834). */
+pskel_status_t pskel_device_info(int32_t device, char* name, size_t name_cap,
+                                 size_t* mem_free, size_t* mem_total, int32_t* num_cus);
+
+/* ---- device residency (replaces ArrayBase::deviceAlloc/deviceFree,
+ *      include/PSkelArray.hpp:72-95, and copyToDevice/copyToHost/copyFromDevice,
+ *      include/PSkelArray.hpp:469-608).  Buffers persist across runs: sweeps
+ *      ping-pong between them with no host copies.  Both buffers are zeroed at
+ *      creation; the reference leaves the device output uninitialised
+ *      (include/PSkelStencil.hpp:1076). */
+pskel_status_t pskel_grid_create(const pskel_grid_desc_t* desc, pskel_grid_t** out);
+pskel_status_t pskel_grid_destroy(pskel_grid_t* grid);
+/* Copy a host array (tightly packed, Array2D/Array3D layout) into the INPUT or
+ * OUTPUT buffer.  Uploading INPUT makes it the current state (parity 0).
+ * host_dtype: the storage type (raw copy) or F32/F64 (converted, rounding to
+ * nearest even for BF16/F16 storage). */
+pskel_status_t pskel_grid_upload(pskel_grid_t* grid, int32_t which, const void* host,
+                                 int32_t host_dtype);
+/* Copy the current state (the result of the last run) into a host array. */
+pskel_status_t pskel_grid_download(pskel_grid_t* grid, void* host, int32_t host_dtype);
+/* Copy the previous ping-pong state (the state one sweep before the result
+ * when the last launch was a single sweep). */
+pskel_status_t pskel_grid_download_previous(pskel_grid_t* grid, void* host, int32_t host_dtype);
+pskel_status_t pskel_grid_info(const pskel_grid_t* grid, pskel_grid_info_t* info);
+
+/* ---- the hot path: StencilBase::runIterativeGPU (include/PSkelStencil.hpp:1047-1097)
+ *      -> runCUDA (:1765-1792) -> stencilTilingCU (:572-652).  Runs `iterations`
+ *      sweeps from the current state.  Successive calls resume where the last one
+ *      stopped, so run(k) then run(n-k) is bit-identical to run(n).
+ *      iterations == 0 leaves the state unchanged (the reference's output := input,
+ *      include/PSkelStencil.hpp:932). */
+pskel_status_t pskel_run_iterative(pskel_grid_t* grid, const pskel_stencil_desc_t* desc,
+                                   size_t iterations, const pskel_launch_cfg_t* cfg,
+                                   pskel_run_stats_t* stats);
+
+/* FP64 ||current - previous||_2 over every cell (the convergence test the
+ * reference leaves to the caller). */
+pskel_status_t pskel_grid_residual(pskel_grid_t* grid, double* out);
+
+/* ---- utilities ------------------------------------------------------------ */
+/* Plain device-to-device copy kernel (the bandwidth yardstick for roofline
+ * calibration); ms (may be NULL) receives its event-timed duration. */
+pskel_status_t pskel_device_copy(void* dst, const void* src, size_t bytes, void* stream,
+                                 double* ms);
+/* Device buffers for tools and tests (hipMalloc / hipFree / hipMemcpy). */
+pskel_status_t pskel_device_alloc(size_t bytes, void** out);
+pskel_status_t pskel_device_free(void* ptr);
+pskel_status_t pskel_memcpy(void* dst, const void* src, size_t bytes, int32_t to_device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSKEL_AMD_H */

@@ -1,0 +1,139 @@
+// Device-side helpers shared by the kernel translation units (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pskel_internal.hpp"
+
+namespace pskel {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Storage formats.  Arithmetic is always f32; 16-bit formats store (u - off)
+// and round to nearest even on the way out.
+// ---------------------------------------------------------------------------
+template <int ST>
+struct Stor;
+
+template <>
+struct Stor<PSKEL_DTYPE_F32> {
+    using T = float;
+    __device__ __forceinline__ static float dec(T v, float) { return v; }
+    __device__ __forceinline__ static float dec_raw(T v) { return v; }
+    __device__ __forceinline__ static T enc(float v, float) { return v; }
+};
+
+template <>
+struct Stor<PSKEL_DTYPE_BF16> {
+    using T = uint16_t;
+    __device__ __forceinline__ static float dec_raw(T v) { return __uint_as_float((uint32_t)v << 16); }
+    __device__ __forceinline__ static float dec(T v, float off) { return dec_raw(v) + off; }
+    __device__ __forceinline__ static T enc(float v, float off)
+    {
+        uint32_t u = __float_as_uint(v - off);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (T)((u >> 16) | 0x40u);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (T)(u >> 16);
+    }
+};
+
+template <>
+struct Stor<PSKEL_DTYPE_F16> {
+    using T = uint16_t;
+    __device__ __forceinline__ static float dec_raw(T v) { return __half2float(__ushort_as_half(v)); }
+    __device__ __forceinline__ static float dec(T v, float off) { return dec_raw(v) + off; }
+    __device__ __forceinline__ static T enc(float v, float off)
+    {
+        return __half_as_ushort(__float2half_rn(v - off));
+    }
+};
+
+// Vector load/store of N consecutive elements of T (N * sizeof(T) in {2,4,8,16} bytes).
+template <typename T, int N>
+struct Vec {
+    T v[N];
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void load_vec(const T* __restrict__ p, T (&out)[N])
+{
+    constexpr int B = N * (int)sizeof(T);
+    if constexpr (B == 16) {
+        uint4 x = *reinterpret_cast<const uint4*>(p);
+        __builtin_memcpy(out, &x, 16);
+    } else if constexpr (B == 8) {
+        uint2 x = *reinterpret_cast<const uint2*>(p);
+        __builtin_memcpy(out, &x, 8);
+    } else if constexpr (B == 4) {
+        uint32_t x = *reinterpret_cast<const uint32_t*>(p);
+        __builtin_memcpy(out, &x, 4);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) out[i] = p[i];
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&in)[N])
+{
+    constexpr int B = N * (int)sizeof(T);
+    if constexpr (B == 16) {
+        uint4 x;
+        __builtin_memcpy(&x, in, 16);
+        *reinterpret_cast<uint4*>(p) = x;
+    } else if constexpr (B == 8) {
+        uint2 x;
+        __builtin_memcpy(&x, in, 8);
+        *reinterpret_cast<uint2*>(p) = x;
+    } else if constexpr (B == 4) {
+        uint32_t x;
+        __builtin_memcpy(&x, in, 4);
+        *reinterpret_cast<uint32_t*>(p) = x;
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) p[i] = in[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic FP64 block reduction: wave butterfly, then waves in order.
+// Every thread of the block must call it.  Returns the block sum in thread 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+template <int NWAVES>
+__device__ __forceinline__ double block_sum(double v, double* scratch /* NWAVES entries */)
+{
+    const int lane = threadIdx.x % kWave;
+    const int wv = threadIdx.x / kWave;
+    v = wave_sum(v);
+    if (lane == 0) scratch[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NWAVES; ++i) s += scratch[i];
+    }
+    return s;
+}
+
+// Map a linear block id so that consecutive logical ids land on the same XCD:
+// the dispatcher hands blocks to the 8 XCDs round-robin, so logical neighbours
+// (which share halo lines) would otherwise sit in 8 different L2s.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks)
+{
+    constexpr uint32_t kXcd = 8;
+    if (nblocks % kXcd != 0) return bid;
+    return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
+}
+
+}  // namespace pskel

@@ -1,0 +1,345 @@
+// 2D float stencil sweeps for gfx950.
+//
+// Reference contract: stencilTilingCU (include/PSkelStencil.hpp:572-652) calls
+// the user kernel for every interior cell w in [r, W-r), h in [r, H-r); the ring
+// is never written.  Every kernel here computes the same expression in the same
+// order (no contraction: built with -ffp-contract=off), so all variants are
+// bit-identical to each other and to runIterativeCPU.
+//
+//   k2d_plain  one lane per cell, the caller's (bx, by) block: the runCUDA shape.
+//   k2d_tuned  wave64 x float4 row segments, RY rows per lane, neighbours along w
+//              exchanged across lanes, all row loads issued before any math.
+//   k2d_ttile  overlapped temporal tile: K sweeps in LDS per launch, K-cell halo
+//              recomputed redundantly (the rewrite of PSkelStencilTiling).
+#include "k_common.hpp"
+
+namespace pskel {
+
+namespace {
+
+constexpr int KIND_SCALED = PSKEL_KIND_SCALED_SUM;
+constexpr int KIND_WEIGHTED = PSKEL_KIND_WEIGHTED_SUM;
+
+// ---------------------------------------------------------------------------
+// plain
+// ---------------------------------------------------------------------------
+template <int KIND, int NP>
+__global__ __launch_bounds__(1024) void k2d_plain(Sweep2D s, Points p)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = blockIdx.y * blockDim.y + threadIdx.y;
+    const int r = s.radius;
+    if (w < r || w >= s.W - r || h < r || h >= s.H - r) return;
+    const float* __restrict__ c = (const float*)s.src + (size_t)h * s.pitch + w;
+    const int np = NP > 0 ? NP : p.np;
+    const ptrdiff_t pitch = (ptrdiff_t)s.pitch;
+    float acc;
+    if constexpr (KIND == KIND_SCALED) {
+        acc = c[p.dh[0] * pitch + p.dw[0]];
+#pragma unroll
+        for (int i = 1; i < (NP > 0 ? NP : kMaxPoints); ++i) {
+            if (NP == 0 && i >= np) break;
+            acc = acc + c[p.dh[i] * pitch + p.dw[i]];
+        }
+        acc = p.coeff * acc;
+    } else {
+        acc = p.wt[0] * c[p.dh[0] * pitch + p.dw[0]];
+#pragma unroll
+        for (int i = 1; i < (NP > 0 ? NP : kMaxPoints); ++i) {
+            if (NP == 0 && i >= np) break;
+            acc = acc + p.wt[i] * c[p.dh[i] * pitch + p.dw[i]];
+        }
+    }
+    ((float*)s.dst)[(size_t)h * s.pitch + w] = acc;
+}
+
+template <int KIND>
+hipError_t plain_dispatch(const Sweep2D& s, const Points& p, dim3 grid, dim3 block, hipStream_t st)
+{
+    switch (p.np) {
+    case 4: hipLaunchKernelGGL((k2d_plain<KIND, 4>), grid, block, 0, st, s, p); break;
+    case 5: hipLaunchKernelGGL((k2d_plain<KIND, 5>), grid, block, 0, st, s, p); break;
+    case 8: hipLaunchKernelGGL((k2d_plain<KIND, 8>), grid, block, 0, st, s, p); break;
+    case 9: hipLaunchKernelGGL((k2d_plain<KIND, 9>), grid, block, 0, st, s, p); break;
+    default: hipLaunchKernelGGL((k2d_plain<KIND, 0>), grid, block, 0, st, s, p); break;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// point evaluation over a register window (radius 1 patterns)
+// ---------------------------------------------------------------------------
+// Window access: row i (0..2 = h-1..h+1), column j in -1..4 relative to the
+// lane's float4 (-1: left halo, 4: right halo).
+struct Win {
+    const float4* v;  // 3 rows
+    const float* lf;  // 3 left halos
+    const float* rt;  // 3 right halos
+    __device__ __forceinline__ float at(int i, int j) const
+    {
+        if (j < 0) return lf[i];
+        if (j > 3) return rt[i];
+        const float4 x = v[i];
+        return j == 0 ? x.x : j == 1 ? x.y : j == 2 ? x.z : x.w;
+    }
+};
+
+template <int PAT>
+__device__ __forceinline__ float eval_pat(const Win& W, int j, const Points& p)
+{
+    if constexpr (PAT == PAT_J5) {
+        float acc = W.at(0, j) + W.at(2, j);
+        acc = acc + W.at(1, j - 1);
+        acc = acc + W.at(1, j + 1);
+        return p.coeff * acc;
+    } else {  // PAT_BOX9, row-major weighted sum
+        float acc = p.wt[0] * W.at(0, j - 1);
+        acc = acc + p.wt[1] * W.at(0, j);
+        acc = acc + p.wt[2] * W.at(0, j + 1);
+        acc = acc + p.wt[3] * W.at(1, j - 1);
+        acc = acc + p.wt[4] * W.at(1, j);
+        acc = acc + p.wt[5] * W.at(1, j + 1);
+        acc = acc + p.wt[6] * W.at(2, j - 1);
+        acc = acc + p.wt[7] * W.at(2, j);
+        acc = acc + p.wt[8] * W.at(2, j + 1);
+        return acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tuned: 64 lanes x float4 = 256 columns per wave, RY rows per lane.
+// block = (64, NWY): the NWY waves stack along h.
+// ---------------------------------------------------------------------------
+template <int PAT, int RY>
+__global__ __launch_bounds__(256) void k2d_tuned(Sweep2D s, Points p)
+{
+    const int lane = threadIdx.x;
+    const int w0 = blockIdx.x * 256;
+    const int w = w0 + lane * 4;
+    const int hb = (blockIdx.y * blockDim.y + threadIdx.y) * RY;  // first output row
+    const int H = s.H, Wd = s.W, r = s.radius;
+    const float* __restrict__ src = (const float*)s.src;
+    float* __restrict__ dst = (float*)s.dst;
+    const size_t pitch = s.pitch;
+    const bool col_live = w < Wd;
+
+    float4 v[RY + 2];
+    float lfl[RY + 2], rtl[RY + 2];
+#pragma unroll
+    for (int i = 0; i < RY + 2; ++i) {
+        const int h = hb - 1 + i;
+        v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        lfl[i] = 0.f;
+        rtl[i] = 0.f;
+        if (h >= 0 && h < H) {
+            const float* row = src + (size_t)h * pitch;
+            if (col_live) v[i] = *reinterpret_cast<const float4*>(row + w);
+            if (lane == 0 && w0 > 0) lfl[i] = row[w0 - 1];
+            if (lane == kWave - 1 && w0 + 256 < Wd) rtl[i] = row[w0 + 256];
+        }
+    }
+    float lf[RY + 2], rt[RY + 2];
+#pragma unroll
+    for (int i = 0; i < RY + 2; ++i) {
+        const float up = __shfl_up(v[i].w, 1, kWave);
+        const float dn = __shfl_down(v[i].x, 1, kWave);
+        lf[i] = lane == 0 ? lfl[i] : up;
+        rt[i] = lane == kWave - 1 ? rtl[i] : dn;
+    }
+    if (!col_live) return;
+    const bool full = (w >= r) && (w + 3 < Wd - r);
+#pragma unroll
+    for (int i = 1; i <= RY; ++i) {
+        const int h = hb - 1 + i;
+        if (h < r || h >= H - r) continue;
+        const Win win{&v[i - 1], &lf[i - 1], &rt[i - 1]};
+        float o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[j] = eval_pat<PAT>(win, j, p);
+        float* row = dst + (size_t)h * pitch + w;
+        if (full) {
+            *reinterpret_cast<float4*>(row) = make_float4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (w + j >= r && w + j < Wd - r) row[j] = o[j];
+        }
+    }
+}
+
+template <int PAT>
+hipError_t tuned_dispatch(const Sweep2D& s, const Points& p, int ry, int nwy, hipStream_t st)
+{
+    const dim3 block(kWave, nwy);
+#define PSKEL_TUNED_CASE(RYV)                                                                    \
+    case RYV: {                                                                                  \
+        const dim3 grid((s.W + 255) / 256, (s.H + nwy * RYV - 1) / (nwy * RYV));                 \
+        hipLaunchKernelGGL((k2d_tuned<PAT, RYV>), grid, block, 0, st, s, p);                     \
+        break;                                                                                   \
+    }
+    switch (ry) {
+        PSKEL_TUNED_CASE(1)
+        PSKEL_TUNED_CASE(2)
+        PSKEL_TUNED_CASE(4)
+        PSKEL_TUNED_CASE(8)
+        PSKEL_TUNED_CASE(16)
+    default: return hipErrorInvalidValue;
+    }
+#undef PSKEL_TUNED_CASE
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// temporal tile: core CX x CY cells, loaded region (CX + 8) x (CY + 2K) with a
+// 4-column aligned halo (K <= 4), K sweeps in LDS ping-pong.
+// Cells outside the interior are never computed: at level j an LDS buffer holds
+// the ring of the global buffer that level j lives in (level parity alternates
+// between src and ring_other), so halo reads near the grid edge see exactly what
+// a single sweep would see.
+// ---------------------------------------------------------------------------
+template <int PAT, int K, int CX, int CY>
+__global__ __launch_bounds__(256) void k2d_ttile(Sweep2D s, Points p)
+{
+    constexpr int LX = CX + 8;
+    constexpr int LY = CY + 2 * K;
+    constexpr int NT = 256;
+    __shared__ float A[LY * LX];
+    __shared__ float B[LY * LX];
+    const int tid = threadIdx.x;
+    const int w0 = blockIdx.x * CX;            // core origin
+    const int h0 = blockIdx.y * CY;
+    const int gx0 = w0 - 4, gy0 = h0 - K;      // loaded region origin
+    const int H = s.H, Wd = s.W, r = s.radius;
+    const size_t pitch = s.pitch;
+    const float* __restrict__ src = (const float*)s.src;
+    const float* __restrict__ oth = (const float*)s.ring_other;
+
+    // load: float4 granules, LX/4 per row
+    constexpr int G = LX / 4;
+    for (int q = tid; q < G * LY; q += NT) {
+        const int y = q / G, x4 = (q - y * G) * 4;
+        const int gh = gy0 + y, gw = gx0 + x4;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+        if (gh >= 0 && gh < H && gw >= 0 && gw < (int)pitch) {
+            a = *reinterpret_cast<const float4*>(src + (size_t)gh * pitch + gw);
+            const bool ring_row = gh < r || gh >= H - r;
+            const bool ring_col = gw < r || gw + 3 >= Wd - r;
+            if (ring_row || ring_col) b = *reinterpret_cast<const float4*>(oth + (size_t)gh * pitch + gw);
+        }
+        *reinterpret_cast<float4*>(&A[y * LX + x4]) = a;
+        *reinterpret_cast<float4*>(&B[y * LX + x4]) = b;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 1; j <= K; ++j) {
+        const float* __restrict__ P = (j & 1) ? A : B;
+        float* __restrict__ Q = (j & 1) ? B : A;
+        const int xlo = 4 - (K - j), xhi = 4 + CX + (K - j);
+        const int ylo = j, yhi = LY - j;
+        const int nx = xhi - xlo, ny = yhi - ylo;
+        for (int q = tid; q < nx * ny; q += NT) {
+            const int y = ylo + q / nx, x = xlo + q % nx;
+            const int gh = gy0 + y, gw = gx0 + x;
+            if (gh < r || gh >= H - r || gw < r || gw >= Wd - r) continue;
+            const float* c = P + y * LX + x;
+            float acc;
+            if constexpr (PAT == PAT_J5) {
+                acc = c[-LX] + c[LX];
+                acc = acc + c[-1];
+                acc = acc + c[1];
+                acc = p.coeff * acc;
+            } else {
+                acc = p.wt[0] * c[-LX - 1];
+                acc = acc + p.wt[1] * c[-LX];
+                acc = acc + p.wt[2] * c[-LX + 1];
+                acc = acc + p.wt[3] * c[-1];
+                acc = acc + p.wt[4] * c[0];
+                acc = acc + p.wt[5] * c[1];
+                acc = acc + p.wt[6] * c[LX - 1];
+                acc = acc + p.wt[7] * c[LX];
+                acc = acc + p.wt[8] * c[LX + 1];
+            }
+            Q[y * LX + x] = acc;
+        }
+        __syncthreads();
+    }
+
+    const float* __restrict__ R = (K & 1) ? B : A;
+    float* __restrict__ dst = (float*)s.dst;
+    for (int q = tid; q < CX * CY; q += NT) {
+        const int y = q / CX, x = q % CX;
+        const int gh = h0 + y, gw = w0 + x;
+        if (gh < r || gh >= H - r || gw < r || gw >= Wd - r) continue;
+        dst[(size_t)gh * pitch + gw] = R[(y + K) * LX + (x + 4)];
+    }
+}
+
+template <int PAT, int K>
+hipError_t ttile_launch_k(const Sweep2D& s, const Points& p, int cy, hipStream_t st)
+{
+    constexpr int CX = 120;
+    const dim3 block(256);
+    switch (cy) {
+    case 16: {
+        const dim3 grid((s.W + CX - 1) / CX, (s.H + 15) / 16);
+        hipLaunchKernelGGL((k2d_ttile<PAT, K, CX, 16>), grid, block, 0, st, s, p);
+        break;
+    }
+    case 32: {
+        const dim3 grid((s.W + CX - 1) / CX, (s.H + 31) / 32);
+        hipLaunchKernelGGL((k2d_ttile<PAT, K, CX, 32>), grid, block, 0, st, s, p);
+        break;
+    }
+    case 64: {
+        const dim3 grid((s.W + CX - 1) / CX, (s.H + 63) / 64);
+        hipLaunchKernelGGL((k2d_ttile<PAT, K, CX, 64>), grid, block, 0, st, s, p);
+        break;
+    }
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int PAT>
+hipError_t ttile_dispatch(const Sweep2D& s, const Points& p, int k, int cy, hipStream_t st)
+{
+    switch (k) {
+    case 2: return ttile_launch_k<PAT, 2>(s, p, cy, st);
+    case 3: return ttile_launch_k<PAT, 3>(s, p, cy, st);
+    case 4: return ttile_launch_k<PAT, 4>(s, p, cy, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_2d_plain_f32(const Sweep2D& s, const Points& p, int bx, int by, hipStream_t st)
+{
+    const dim3 block(bx, by);
+    const dim3 grid((s.W + bx - 1) / bx, (s.H + by - 1) / by);
+    if (p.kind == PSKEL_KIND_SCALED_SUM) return plain_dispatch<KIND_SCALED>(s, p, grid, block, st);
+    return plain_dispatch<KIND_WEIGHTED>(s, p, grid, block, st);
+}
+
+hipError_t launch_2d_tuned_f32(const Sweep2D& s, const Points& p, int pattern,
+                               const pskel_launch_cfg_t& cfg, hipStream_t st)
+{
+    const int ry = cfg.tile_y > 0 ? cfg.tile_y : 4;
+    const int nwy = cfg.tile_z > 0 ? cfg.tile_z : 4;
+    if (nwy < 1 || nwy > 4) return hipErrorInvalidValue;
+    if (pattern == PAT_J5) return tuned_dispatch<PAT_J5>(s, p, ry, nwy, st);
+    if (pattern == PAT_BOX9) return tuned_dispatch<PAT_BOX9>(s, p, ry, nwy, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_2d_ttile_f32(const Sweep2D& s, const Points& p, int pattern, int k,
+                               const pskel_launch_cfg_t& cfg, hipStream_t st)
+{
+    const int cy = cfg.tile_y > 0 ? cfg.tile_y : 32;
+    if (pattern == PAT_J5) return ttile_dispatch<PAT_J5>(s, p, k, cy, st);
+    if (pattern == PAT_BOX9) return ttile_dispatch<PAT_BOX9>(s, p, k, cy, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace pskel

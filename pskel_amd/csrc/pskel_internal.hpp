@@ -1,0 +1,145 @@
+// Internal declarations shared by the host layer and the kernel translation units.
+// Nothing here is part of the public C ABI (include/pskel_amd.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/pskel_amd.h"
+
+namespace pskel {
+
+// ---------------------------------------------------------------------------
+// Error plumbing: every C entry point returns a status and records a message.
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+pskel_status_t fail(pskel_status_t st, const std::string& msg);
+pskel_status_t hip_fail(hipError_t e, const char* what);
+
+#define PSKEL_HIP(call)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) return ::pskel::hip_fail(e_, #call);             \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Stencil descriptor as the kernels see it (kernel argument, by value).
+// ---------------------------------------------------------------------------
+constexpr int kMaxPoints = PSKEL_MAX_POINTS;
+
+struct Points {
+    int32_t np;
+    int32_t radius;
+    int32_t kind;
+    int32_t boundary;
+    float coeff;
+    float halo;
+    int32_t dh[kMaxPoints];
+    int32_t dw[kMaxPoints];
+    int32_t dd[kMaxPoints];
+    float wt[kMaxPoints];
+};
+
+// Point-set patterns the tuned kernels are specialised for.  The tuned kernels
+// accumulate in exactly this order, so a descriptor only qualifies when its
+// points are listed in this order (life: any order, the sum is integer).
+enum Pattern : int {
+    PAT_GENERIC = 0,
+    PAT_J5 = 1,   // scaled sum (-1,0) (1,0) (0,-1) (0,1)
+    PAT_BOX9 = 2, // weighted sum over the 3x3 box, row-major
+    PAT_LIFE8 = 3,// life over the 8 Moore neighbours
+    PAT_J7 = 4,   // 3D scaled sum (-1,0,0) (1,0,0) (0,-1,0) (0,1,0) (0,0,-1) (0,0,1)
+};
+
+int classify(const pskel_stencil_desc_t& d, int ndim);
+int desc_radius(const pskel_stencil_desc_t& d, int ndim);
+void fill_points(Points& p, const pskel_stencil_desc_t& d, int ndim);
+
+// ---------------------------------------------------------------------------
+// Grid: device-resident ping-pong pair plus an optional spare buffer.
+// 2D storage is row-pitched: cell (h, w) at h*pitch + w.
+// 3D storage pitches the contiguous d axis: (h*W + w)*pitch + d.
+// pitch (in elements) keeps every row 256-byte aligned and >= the tuned tile.
+// ---------------------------------------------------------------------------
+struct Grid {
+    int ndim = 2;
+    int storage = PSKEL_DTYPE_F32;
+    int elem = 4;
+    size_t W = 0, H = 0, D = 1;
+    size_t pitch = 0;   // elements per contiguous row
+    size_t nrows = 0;   // 2D: H, 3D: H*W
+    size_t bytes = 0;   // per buffer
+    float store_offset = 0.0f;
+    int device = 0;
+
+    void* buf[3] = {nullptr, nullptr, nullptr}; // [cur, other, spare]
+    int parity = 0;       // which uploaded buffer's ring `cur` carries (0 = input)
+    int spare_parity = -1;// ring carried by the spare (-1: not allocated / unknown)
+
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    double* d_partials = nullptr;   // per-block FP64 residual partials
+    size_t partials_cap = 0;
+    double* d_res = nullptr;        // reduced residuals, one per residual sweep
+    size_t res_cap = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Launch plumbing shared by the kernel TUs.
+// ---------------------------------------------------------------------------
+struct Sweep2D {
+    const void* src;
+    void* dst;
+    const void* ring_other; // buffer carrying the other parity's ring (temporal tiles)
+    int W, H;
+    size_t pitch;
+    int radius;
+};
+
+struct Sweep3D {
+    const void* src;
+    void* dst;
+    int W, H, D;
+    size_t pitch;
+    int radius;
+    float store_offset;
+    double* partials;   // non-null: accumulate sum((new-old)^2) per block
+};
+
+// Planned launch shape for a residual-reduction: number of blocks a 3D sweep uses.
+size_t sweep3d_blocks(const Grid& g, int pattern, const pskel_launch_cfg_t& cfg);
+
+// 2D float (f32) sweeps; return hipSuccess or the launch error.
+hipError_t launch_2d_plain_f32(const Sweep2D& s, const Points& p, int bx, int by, hipStream_t st);
+hipError_t launch_2d_tuned_f32(const Sweep2D& s, const Points& p, int pattern,
+                               const pskel_launch_cfg_t& cfg, hipStream_t st);
+hipError_t launch_2d_ttile_f32(const Sweep2D& s, const Points& p, int pattern, int k,
+                               const pskel_launch_cfg_t& cfg, hipStream_t st);
+
+// 2D life (u8) sweeps.
+hipError_t launch_life_plain(const Sweep2D& s, const Points& p, int bx, int by, hipStream_t st);
+hipError_t launch_life_tuned(const Sweep2D& s, const pskel_launch_cfg_t& cfg, hipStream_t st);
+hipError_t launch_life_ttile(const Sweep2D& s, int k, const pskel_launch_cfg_t& cfg, hipStream_t st);
+
+// 3D sweeps for every storage type.
+hipError_t launch_3d_plain(const Sweep3D& s, const Points& p, int storage, int bx, int by,
+                           hipStream_t st);
+hipError_t launch_3d_tuned(const Sweep3D& s, const Points& p, int storage,
+                           const pskel_launch_cfg_t& cfg, hipStream_t st);
+size_t plain3d_blocks(const Sweep3D& s, int bx, int by);
+size_t tuned3d_blocks(const Sweep3D& s, int storage, const pskel_launch_cfg_t& cfg);
+
+// Deterministic second-stage reduction: out = sqrt(sum(partials[0..n))).
+hipError_t launch_reduce_partials(const double* partials, size_t n, double* out, hipStream_t st);
+
+// Utilities.
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
+hipError_t launch_fill_zero(void* dst, size_t bytes, hipStream_t st);
+hipError_t launch_copy_ring2d(const Sweep2D& s, int elem, hipStream_t st);
+
+}  // namespace pskel
