@@ -1,0 +1,290 @@
+"""Benchmark of the iterative stencil hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload jacobi3d|life|jacobi2d|diffusion9]
+
+A step is one runIterativeGPU of the workload: the configured number of sweeps
+over the whole grid, with both buffers already resident in HBM when the timed
+region starts (uploads happen before it).  Prints ONE JSON line.
+
+value        cell-updates/s of the headline workload (tuned kernels)
+roofline     the dominant kernel: algorithmic bytes per launch (2 x storage
+             bytes per cell, one read + one write per sweep) over its average
+             launch time, from HIP events on the stream the kernels run on
+cpu_baseline the CPU oracle (OpenMP restatement of runIterativeCPU) on a
+             bounded sample of the same workload, on this host's cores
+workloads    the other measured forms (plain vs tuned, GoL, 16-bit storage)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (shape, storage, stencil, sweeps per step, generator)
+    "jacobi3d": dict(shape=(512, 512, 512), storage="f32", sweeps=1000, label="jacobi3d-7pt 512^3 f32 x1000"),
+    "jacobi3d_bf16": dict(shape=(512, 512, 512), storage="bf16", sweeps=1000, label="jacobi3d-7pt 512^3 bf16-storage x1000"),
+    "jacobi3d_f16": dict(shape=(512, 512, 512), storage="f16", sweeps=1000, label="jacobi3d-7pt 512^3 f16-storage x1000"),
+    "life": dict(shape=(8194, 8194), storage="u8", sweeps=500, label="life B3/S23 8192^2 u8 x500"),
+    "jacobi2d": dict(shape=(2048, 2048), storage="f32", sweeps=100, label="jacobi2d-5pt 2048^2 f32 x100"),
+    "diffusion9": dict(shape=(4096, 4096), storage="f32", sweeps=1000, label="diffusion9 4096^2 f32 x1000"),
+}
+
+# Default tuned launch shapes (overridden by --tile / --time-tile).
+TUNED = {
+    "jacobi3d": dict(tile=(4, 1, 128)),
+    "jacobi3d_bf16": dict(tile=(4, 1, 32)),
+    "jacobi3d_f16": dict(tile=(4, 1, 64)),
+    "life": dict(tile=(0, 16, 0), time_tile=4),
+    "jacobi2d": dict(tile=(0, 16, 0), time_tile=4),
+    "diffusion9": dict(tile=(0, 16, 0), time_tile=3),
+}
+
+
+def stencil_of(name):
+    from pskel_amd import kernels as K
+
+    if name.startswith("jacobi3d"):
+        return K.jacobi3d()
+    return {"life": K.life, "jacobi2d": K.jacobi2d, "diffusion9": K.diffusion9}[name]()
+
+
+def field_of(name):
+    from pskel_amd import inputs
+
+    if name.startswith("jacobi3d"):
+        return inputs.jacobi3d_field(512, seed=4)
+    if name == "life":
+        return inputs.life_field(8192, seed=2)
+    if name == "jacobi2d":
+        return inputs.jacobi2d_field(2048, 2048, seed=1)
+    return inputs.blobs_field(4096, seed=3)
+
+
+def store_offset_of(name):
+    # 16-bit storage holds u - 0.5: the interior relaxes towards the mean 0.5,
+    # so the stored deviations are small and keep more relative precision
+    return 0.5 if name in ("jacobi3d_bf16", "jacobi3d_f16") else 0.0
+
+
+def measure(name, variant, steps, warmup, field=None, run_kw=None):
+    """Run `warmup` + `steps` runIterativeGPU steps on a resident grid.
+    Returns (cell-updates/s from the host wall clock, per-step details)."""
+    from pskel_amd import api
+
+    w = WORKLOADS[name]
+    kw = dict(run_kw or {})
+    if variant == "tuned" and not kw:
+        kw = dict(TUNED[name])
+    f = field_of(name) if field is None else field
+    s = stencil_of(name)
+    with api.Grid(w["shape"], w["storage"], store_offset=store_offset_of(name)) as g:
+        g.upload_pair(f)
+        for _ in range(warmup):
+            g.run(s, w["sweeps"], variant=variant, **kw)
+        api.synchronize()
+        t0 = time.perf_counter()
+        dev_ms = 0.0
+        launches = 0
+        for _ in range(steps):
+            st = g.run(s, w["sweeps"], variant=variant, **kw)
+            dev_ms += st["elapsed_ms"]
+            launches += st["launches"]
+        api.synchronize()
+        wall = time.perf_counter() - t0
+    cells = int(np.prod(w["shape"]))
+    cu = cells * w["sweeps"] * steps
+    elem = {"f32": 4, "bf16": 2, "f16": 2, "u8": 1}[w["storage"]]
+    return {
+        "workload": w["label"],
+        "variant": variant,
+        "launch": {k: v for k, v in kw.items()},
+        "cell_updates_per_s": cu / wall,
+        "ms_per_step": wall * 1e3 / steps,
+        "device_ms_per_step": dev_ms / steps,
+        "launches_per_step": launches // max(steps, 1),
+        "avg_launch_ms": dev_ms / max(launches, 1),
+        "sweeps_per_launch": w["sweeps"] * steps / max(launches, 1),
+        "algorithmic_bytes_per_cu": 2 * elem,
+        "hbm_roofline_frac": (cu / wall) * 2 * elem / (HBM_PEAK_GBS * 1e9),
+        "cells": cells,
+    }
+
+
+def cpu_baseline(name, seconds_target=12.0):
+    """The oracle (runIterativeCPU restatement, OpenMP) on a bounded sample:
+    the full grid for as many sweeps as fit about `seconds_target`."""
+    import oracle
+
+    w = WORKLOADS[name]
+    f = field_of(name)
+    s = stencil_of(name)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or oracle.num_procs()
+    threads = min(threads, 16)
+    # calibrate with 2 sweeps, then time the sample
+    run = (lambda n: oracle.run_3d(s, f, n, threads=threads)) if len(w["shape"]) == 3 else \
+        (lambda n: oracle.run_2d(s, f, n, threads=threads))
+    run(2)
+    per = oracle.last_seconds() / 2
+    n = max(2, min(w["sweeps"], int(seconds_target / max(per, 1e-6))))
+    n -= n % 2
+    run(n)
+    secs = oracle.last_seconds()
+    cells = int(np.prod(w["shape"]))
+    return {
+        "value": cells * n / secs,
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{w['label'].split(' x')[0]}, {n} of {w['sweeps']} sweeps ({secs:.1f} s, f32 OpenMP)",
+    }
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{name}.txt")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def copy_calibration(mib, reps):
+    """Plain 16-byte-vector copy of `mib` MiB, `reps` times: the yardstick that
+    converts FETCH_SIZE / WRITE_SIZE readings to bytes, and the achievable
+    copy bandwidth of this device."""
+    import ctypes as C
+
+    from pskel_amd import _lib as L
+
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    L.check(L.lib().pskel_device_alloc(n, C.byref(a)))
+    L.check(L.lib().pskel_device_alloc(n, C.byref(b)))
+    try:
+        ms = C.c_double()
+        L.check(L.lib().pskel_device_copy(b, a, n, None, C.byref(ms)))  # warm
+        times = []
+        for _ in range(reps):
+            L.check(L.lib().pskel_device_copy(b, a, n, None, C.byref(ms)))
+            times.append(ms.value)
+    finally:
+        L.lib().pskel_device_free(a)
+        L.lib().pskel_device_free(b)
+    best = min(times)
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "reps": reps,
+                      "best_ms": best, "copy_GBps": 2 * n / (best * 1e-3) / 1e9}))
+    return 0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="jacobi3d", choices=sorted(WORKLOADS))
+    ap.add_argument("--variant", default="tuned", choices=["tuned", "plain"])
+    ap.add_argument("--tile", default="", help="tuned tile x,y,z")
+    ap.add_argument("--time-tile", type=int, default=0)
+    ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run the plain copy kernel on two buffers of this size (PMC calibration)")
+    args = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the grid would shard); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from pskel_amd import _lib, api
+
+    if _lib.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+
+    if args.copy_mib:
+        return copy_calibration(args.copy_mib, max(args.steps, 1))
+
+    run_kw = None
+    if args.tile or args.time_tile:
+        run_kw = dict(TUNED[args.workload])
+        if args.tile:
+            run_kw["tile"] = tuple(int(x) for x in args.tile.split(","))
+        if args.time_tile:
+            run_kw["time_tile"] = args.time_tile
+
+    head = measure(args.workload, args.variant, args.steps, args.warmup, run_kw=run_kw)
+    print(f"# headline {head['workload']} {head['variant']}: {head['cell_updates_per_s']:.4g} CU/s "
+          f"{head['hbm_roofline_frac']*100:.1f}% of HBM", file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.no_extra:
+        plan = [("jacobi3d", "plain"), ("jacobi3d_bf16", "tuned"), ("jacobi3d_f16", "tuned"),
+                ("life", "tuned"), ("life", "plain")]
+        for nm, var in plan:
+            if (nm, var) == (args.workload, args.variant):
+                continue
+            r = measure(nm, var, max(1, min(args.steps, 2)), 1)
+            extra[f"{nm}:{var}"] = {k: r[k] for k in ("workload", "cell_updates_per_s", "ms_per_step",
+                                                       "avg_launch_ms", "hbm_roofline_frac", "launch")}
+            print(f"# {nm} {var}: {r['cell_updates_per_s']:.4g} CU/s {r['hbm_roofline_frac']*100:.1f}%",
+                  file=sys.stderr, flush=True)
+
+    bytes_per_launch = head["algorithmic_bytes_per_cu"] * head["cells"] * head["sweeps_per_launch"]
+    achieved = bytes_per_launch / (head["avg_launch_ms"] * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profiles(args.workload),
+        "kernel": "k3d_march" if args.workload.startswith("jacobi3d") and args.variant == "tuned" else None,
+        "algorithmic_bytes_per_launch": bytes_per_launch,
+        "avg_launch_ms": round(head["avg_launch_ms"], 5),
+    }
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.workload)
+
+    info = api.device_info(0)
+    line = {
+        "metric": "cell-updates/s (+% of HBM roofline), Jacobi3D & GoL, tuned vs plain",
+        "value": head["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32": "f32", "bf16": "bf16-storage/f32", "f16": "f16-storage/f32", "u8": "u8"}[
+            WORKLOADS[args.workload]["storage"]],
+        "data": "synthetic (counter-based splitmix64 fields)",
+        "config": {"workload": head["workload"], "variant": head["variant"], "launch": head["launch"],
+                   "hbm_roofline_frac": round(head["hbm_roofline_frac"], 4), "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "workloads": extra,
+        "device": {"arch": info["name"], "cus": info["cus"]},
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

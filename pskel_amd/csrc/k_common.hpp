@@ -32,12 +32,11 @@ struct Stor<PSKEL_DTYPE_BF16> {
     using T = uint16_t;
     __device__ __forceinline__ static float dec_raw(T v) { return __uint_as_float((uint32_t)v << 16); }
     __device__ __forceinline__ static float dec(T v, float off) { return dec_raw(v) + off; }
+    // gfx950 converts natively (v_cvt_pk_bf16_f32, round to nearest even)
     __device__ __forceinline__ static T enc(float v, float off)
     {
-        uint32_t u = __float_as_uint(v - off);
-        if ((u & 0x7fffffffu) > 0x7f800000u) return (T)((u >> 16) | 0x40u);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return (T)(u >> 16);
+        const __bf16 b = (__bf16)(v - off);
+        return __builtin_bit_cast(T, b);
     }
 };
 

@@ -112,7 +112,7 @@ hipError_t plain3d_kind(const Sweep3D& s, const Points& p, dim3 grid, dim3 block
 // ---------------------------------------------------------------------------
 constexpr int kMarchWaves = 4;
 
-template <int ST, int VD, int RW>
+template <int ST, int VD, int RW, bool RES>
 __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hseg, int ntd,
                                                  int ntw, int nseg)
 {
@@ -134,11 +134,9 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
     const int sg = (int)(lid / ((uint32_t)ntd * ntw));
 
     const int W = s.W, H = s.H, D = s.D, r = s.radius;
-    const size_t pitch = s.pitch;
-    const size_t plane = (size_t)W * pitch;
+    const int pitch = (int)s.pitch;
+    const size_t plane = (size_t)W * s.pitch;
     const float off = s.store_offset;
-    const T* __restrict__ in = (const T*)s.src;
-    T* __restrict__ out = (T*)s.dst;
 
     const int d0 = td * TD, w0 = tw * TW;
     const int d = d0 + lane * VD;
@@ -146,105 +144,123 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
     const int h0 = sg * hseg;
     const int h1 = min(h0 + hseg, H);
     const bool dlive = d < D;
+    // every cell of this lane's vector is an interior d (no face, inside the grid)
+    const bool dint = d >= r && d + VD - 1 < D - r;
 
     // halo roles
     const bool hal_wlo = wave == 0 && w0 > 0;              // loads row w0-1
     const bool hal_whi = wave == NW - 1 && w0 + TW < W;     // loads row w0+TW
     const bool hal_dlo = lane == 0 && d0 > 0;               // loads d0-1 for its rows
     const bool hal_dhi = lane == kWave - 1 && d0 + TD < D;  // loads d0+TD for its rows
+    const int hw_off = (hal_wlo ? -(wave * RW + 1) : RW * (NW - wave)) * pitch;  // w-halo row - own row
+    const int hd_off = hal_dlo ? -1 - lane * VD : TD - lane * VD;                 // d-halo cell - own cell
 
+    bool row_live[RW], row_int[RW];
+#pragma unroll
+    for (int i = 0; i < RW; ++i) {
+        const int w = wr + i;
+        row_live[i] = dlive && w < W;
+        row_int[i] = w >= r && w < W - r;
+    }
+
+    // own cell (h, wr, d) of the plane being loaded / stored
+    const T* __restrict__ pin = (const T*)s.src + (size_t)h0 * plane + (size_t)wr * pitch + d;
+    T* __restrict__ pout = (T*)s.dst + (size_t)h0 * plane + (size_t)wr * pitch + d;
+
+    // register queue along h: cur (h) and nxt (h+1) are resident while the loads
+    // of plane h+2 are in flight, so a plane's load latency overlaps a whole step
     float prv[RW][VD], cur[RW][VD], nxt[RW][VD];
-    T craw[RW][VD], nraw[RW][VD];
-    float hw_c[VD], hw_n[VD];  // w-halo row vector (this wave's role), current / next plane
-    float hd_c[RW], hd_n[RW];  // d-halo scalar per row (lane role), current / next plane
+    T craw[RW][VD], nraw[RW][VD], fraw[RW][VD];
+    T hw_c[VD], hw_n[VD], hw_f[VD];   // w-halo row vector (this wave's role)
+    T hd_c[RW], hd_n[RW], hd_f[RW];   // d-halo cell per row (lane role)
 
-    auto load_plane = [&](int h, float (&v)[RW][VD], T (&raw)[RW][VD], float (&hw)[VD],
-                          float (&hd)[RW]) {
-        const T* base = in + (size_t)h * plane;
+    auto load_plane = [&](const T* p, T (&raw)[RW][VD], T (&hw)[VD], T (&hd)[RW]) {
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
-            const int w = wr + i;
-            if (dlive && w < W) {
-                load_vec<T, VD>(base + (size_t)w * pitch + d, raw[i]);
+            if (row_live[i]) {
+                load_vec<T, VD>(p + i * pitch, raw[i]);
             } else {
 #pragma unroll
                 for (int j = 0; j < VD; ++j) raw[i][j] = T(0);
             }
+            hd[i] = T(0);
+            if ((hal_dlo || hal_dhi) && wr + i < W) hd[i] = p[i * pitch + hd_off];
+        }
+        if (dlive && (hal_wlo || hal_whi)) {
+            load_vec<T, VD>(p + hw_off, hw);
+        } else {
+#pragma unroll
+            for (int j = 0; j < VD; ++j) hw[j] = T(0);
+        }
+    };
+    auto decode = [&](float (&v)[RW][VD], const T (&raw)[RW][VD]) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
 #pragma unroll
             for (int j = 0; j < VD; ++j) v[i][j] = Tr::dec(raw[i][j], off);
-            hd[i] = 0.f;
-            if (w < W) {
-                if (hal_dlo) hd[i] = Tr::dec(base[(size_t)w * pitch + d0 - 1], off);
-                if (hal_dhi) hd[i] = Tr::dec(base[(size_t)w * pitch + d0 + TD], off);
-            }
+    };
+    auto zero_plane = [&](T (&raw)[RW][VD], T (&hw)[VD], T (&hd)[RW]) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            hd[i] = T(0);
+#pragma unroll
+            for (int j = 0; j < VD; ++j) raw[i][j] = T(0);
         }
 #pragma unroll
-        for (int j = 0; j < VD; ++j) hw[j] = 0.f;
-        if (dlive && (hal_wlo || hal_whi)) {
-            const int w = hal_wlo ? w0 - 1 : w0 + TW;
-            T t[VD];
-            load_vec<T, VD>(base + (size_t)w * pitch + d, t);
-#pragma unroll
-            for (int j = 0; j < VD; ++j) hw[j] = Tr::dec(t[j], off);
-        }
+        for (int j = 0; j < VD; ++j) hw[j] = T(0);
     };
 
     if (h0 > 0) {
-        T tr[RW][VD];
-        float thw[VD], thd[RW];
-        load_plane(h0 - 1, prv, tr, thw, thd);
+        T tr[RW][VD], thw[VD], thd[RW];
+        load_plane(pin - plane, tr, thw, thd);
+        decode(prv, tr);
     } else {
 #pragma unroll
         for (int i = 0; i < RW; ++i)
 #pragma unroll
             for (int j = 0; j < VD; ++j) prv[i][j] = 0.f;
     }
-    load_plane(h0, cur, craw, hw_c, hd_c);
+    load_plane(pin, craw, hw_c, hd_c);
+    if (h0 + 1 < H) load_plane(pin + plane, nraw, hw_n, hd_n);
+    else zero_plane(nraw, hw_n, hd_n);
+    decode(cur, craw);
 
     double rr = 0.0;
     int b = 0;
     for (int h = h0; h < h1; ++h) {
-        if (h + 1 < H) {
-            load_plane(h + 1, nxt, nraw, hw_n, hd_n);
-        } else {
-#pragma unroll
-            for (int i = 0; i < RW; ++i)
-#pragma unroll
-                for (int j = 0; j < VD; ++j) { nxt[i][j] = 0.f; nraw[i][j] = T(0); }
-        }
+        if (h + 2 < H && h + 1 < h1) load_plane(pin + 2 * plane, fraw, hw_f, hd_f);
+        else zero_plane(fraw, hw_f, hd_f);
+        decode(nxt, nraw);
         // stage plane h
+        float (*L)[LROW] = lds[b];
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
 #pragma unroll
-            for (int j = 0; j < VD; ++j) lds[b][1 + wave * RW + i][VD + lane * VD + j] = cur[i][j];
-            if (hal_dlo) lds[b][1 + wave * RW + i][VD - 1] = hd_c[i];
-            if (hal_dhi) lds[b][1 + wave * RW + i][VD + TD] = hd_c[i];
+            for (int j = 0; j < VD; ++j) L[1 + wave * RW + i][VD + lane * VD + j] = cur[i][j];
+            if (hal_dlo) L[1 + wave * RW + i][VD - 1] = Tr::dec(hd_c[i], off);
+            if (hal_dhi) L[1 + wave * RW + i][VD + TD] = Tr::dec(hd_c[i], off);
         }
         if (hal_wlo || hal_whi) {
             const int row = hal_wlo ? 0 : TW + 1;
 #pragma unroll
-            for (int j = 0; j < VD; ++j) lds[b][row][VD + lane * VD + j] = hw_c[j];
+            for (int j = 0; j < VD; ++j) L[row][VD + lane * VD + j] = Tr::dec(hw_c[j], off);
         }
         __syncthreads();
 
-        const bool hface = h < r || h >= H - r;
+        const bool hint = h >= r && h < H - r;
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
-            const int w = wr + i;
-            if (!dlive || w >= W) continue;
-            const bool wface = hface || w < r || w >= W - r;
+            if (!row_live[i]) continue;
             const int lr = 1 + wave * RW + i;
             T o[VD];
+            if (hint && row_int[i] && dint) {
+                // interior fast path: no per-cell face tests
 #pragma unroll
-            for (int j = 0; j < VD; ++j) {
-                const int dd = d + j;
-                if (wface || dd < r || dd >= D - r || dd >= D) {
-                    o[j] = craw[i][j];
-                } else {
-                    const float wm = (i > 0) ? cur[i - 1][j] : lds[b][lr - 1][VD + lane * VD + j];
-                    const float wp = (i < RW - 1) ? cur[i + 1][j] : lds[b][lr + 1][VD + lane * VD + j];
-                    const float dm = (j > 0) ? cur[i][j - 1] : lds[b][lr][VD + lane * VD + j - 1];
-                    const float dp = (j < VD - 1) ? cur[i][j + 1] : lds[b][lr][VD + lane * VD + j + 1];
+                for (int j = 0; j < VD; ++j) {
+                    const float wm = (i > 0) ? cur[i - 1][j] : L[lr - 1][VD + lane * VD + j];
+                    const float wp = (i < RW - 1) ? cur[i + 1][j] : L[lr + 1][VD + lane * VD + j];
+                    const float dm = (j > 0) ? cur[i][j - 1] : L[lr][VD + lane * VD + j - 1];
+                    const float dp = (j < VD - 1) ? cur[i][j + 1] : L[lr][VD + lane * VD + j + 1];
                     float acc = prv[i][j] + nxt[i][j];
                     acc = acc + wm;
                     acc = acc + wp;
@@ -252,29 +268,59 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
                     acc = acc + dp;
                     o[j] = Tr::enc(coeff * acc, off);
                 }
-                if (s.partials) {
+            } else {
+                const bool wface = !hint || !row_int[i];
+#pragma unroll
+                for (int j = 0; j < VD; ++j) {
+                    const int dd = d + j;
+                    if (wface || dd < r || dd >= D - r) {
+                        o[j] = craw[i][j];
+                    } else {
+                        const float wm = (i > 0) ? cur[i - 1][j] : L[lr - 1][VD + lane * VD + j];
+                        const float wp = (i < RW - 1) ? cur[i + 1][j] : L[lr + 1][VD + lane * VD + j];
+                        const float dm = (j > 0) ? cur[i][j - 1] : L[lr][VD + lane * VD + j - 1];
+                        const float dp = (j < VD - 1) ? cur[i][j + 1] : L[lr][VD + lane * VD + j + 1];
+                        float acc = prv[i][j] + nxt[i][j];
+                        acc = acc + wm;
+                        acc = acc + wp;
+                        acc = acc + dm;
+                        acc = acc + dp;
+                        o[j] = Tr::enc(coeff * acc, off);
+                    }
+                }
+            }
+            if constexpr (RES) {
+#pragma unroll
+                for (int j = 0; j < VD; ++j) {
                     const double dv = (double)Tr::dec_raw(o[j]) - (double)Tr::dec_raw(craw[i][j]);
                     rr += dv * dv;
                 }
             }
-            store_vec<T, VD>(out + (size_t)h * plane + (size_t)w * pitch + d, o);
+            store_vec<T, VD>(pout + i * pitch, o);
         }
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
             hd_c[i] = hd_n[i];
+            hd_n[i] = hd_f[i];
 #pragma unroll
             for (int j = 0; j < VD; ++j) {
                 prv[i][j] = cur[i][j];
                 cur[i][j] = nxt[i][j];
                 craw[i][j] = nraw[i][j];
+                nraw[i][j] = fraw[i][j];
             }
         }
 #pragma unroll
-        for (int j = 0; j < VD; ++j) hw_c[j] = hw_n[j];
+        for (int j = 0; j < VD; ++j) {
+            hw_c[j] = hw_n[j];
+            hw_n[j] = hw_f[j];
+        }
         b ^= 1;
+        pin += plane;
+        pout += plane;
     }
 
-    if (s.partials) {
+    if constexpr (RES) {
         const double t = block_sum<NW>(rr, red);
         if (threadIdx.x == 0) s.partials[lid] = t;
     }
@@ -311,8 +357,12 @@ hipError_t march_launch(const Sweep3D& s, const Points& p, const MarchShape& m, 
         return hipErrorInvalidValue;
     } else {
         const dim3 grid((unsigned)m.ntd * m.ntw * m.nseg), block(256);
-        hipLaunchKernelGGL((k3d_march<ST, VD, RW>), grid, block, 0, st, s, p.coeff, m.hseg, m.ntd,
-                           m.ntw, m.nseg);
+        if (s.partials)
+            hipLaunchKernelGGL((k3d_march<ST, VD, RW, true>), grid, block, 0, st, s, p.coeff, m.hseg,
+                               m.ntd, m.ntw, m.nseg);
+        else
+            hipLaunchKernelGGL((k3d_march<ST, VD, RW, false>), grid, block, 0, st, s, p.coeff, m.hseg,
+                               m.ntd, m.ntw, m.nseg);
         return hipGetLastError();
     }
 }

@@ -5,13 +5,15 @@ namespace pskel {
 
 namespace {
 
-// 16-byte vector copy, grid-stride; tail bytes handled one by one.
+// 16-byte vector copy, one vector per lane and one 4 KiB chunk per block: the
+// shape that reaches the device's copy bandwidth (a grid-stride loop over a
+// capped grid does not); tail bytes handled one by one.
 __global__ __launch_bounds__(256) void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src,
                                               size_t n16, uint8_t* __restrict__ dtail,
                                               const uint8_t* __restrict__ stail, size_t ntail)
 {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
     if (blockIdx.x == 0 && threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
 }
 
@@ -91,7 +93,9 @@ unsigned copy_blocks(size_t n16)
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 {
     const size_t n16 = bytes / 16, tail = bytes % 16;
-    hipLaunchKernelGGL(k_copy, dim3(copy_blocks(n16)), dim3(256), 0, st, (uint4*)dst, (const uint4*)src,
+    const size_t nb = (n16 + 255) / 256;
+    if (nb > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)(nb ? nb : 1)), dim3(256), 0, st, (uint4*)dst, (const uint4*)src,
                        n16, (uint8_t*)dst + n16 * 16, (const uint8_t*)src + n16 * 16, tail);
     return hipGetLastError();
 }
