@@ -185,9 +185,76 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* grid, const pskel_stencil_desc_
                                    size_t iterations, const pskel_launch_cfg_t* cfg,
                                    pskel_run_stats_t* stats);
 
+/* Record one sweep launched outside the library on the grid's buffers (the
+ * drop-in headers' device path for user stencil kernels with no descriptor):
+ * the previous buffer becomes the current state, as runIterativeGPU's
+ * in/out swap does (include/PSkelStencil.hpp:1078-1082). */
+pskel_status_t pskel_grid_swap(pskel_grid_t* grid);
+
 /* FP64 ||current - previous||_2 over every cell (the convergence test the
  * reference leaves to the caller). */
 pskel_status_t pskel_grid_residual(pskel_grid_t* grid, double* out);
+
+/* ---- launch tuner (replaces runIterativeAutoGPU's GAlib search over an
+ *      analytic objective, include/PSkelStencil.hpp:1865-2076).  Candidates
+ *      are the launch configurations of the kernel family a descriptor maps
+ *      to; a learned cost model (pskel_mlp_t, below) predicts log time per
+ *      sweep from pskel_tuner_features and the tuner takes the argmin.
+ *      Exhaustive search over measured times (pskel_tuner_measure) is the
+ *      fallback and the judge of the model's regret. */
+typedef struct pskel_tune_query {
+    int32_t ndim;
+    int32_t storage;     /* pskel_dtype_t of the grid */
+    size_t width, height, depth;
+    size_t iterations;
+} pskel_tune_query_t;
+
+#define PSKEL_TUNER_MAX_CANDIDATES 256
+#define PSKEL_TUNER_NFEAT 24
+
+typedef struct pskel_mlp pskel_mlp_t;
+
+pskel_status_t pskel_tuner_candidates(const pskel_tune_query_t* q, const pskel_stencil_desc_t* d,
+                                      pskel_launch_cfg_t* out, size_t cap, size_t* n);
+/* Model input of one (query, descriptor, candidate): PSKEL_TUNER_NFEAT floats. */
+pskel_status_t pskel_tuner_features(const pskel_tune_query_t* q, const pskel_stencil_desc_t* d,
+                                    const pskel_launch_cfg_t* c, float* feat);
+/* The candidate with the lowest predicted time.  model NULL: the model installed
+ * with pskel_tuner_set_default_model, else a fixed heuristic per kernel family. */
+pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_query_t* q,
+                                  const pskel_stencil_desc_t* d, pskel_launch_cfg_t* out);
+pskel_status_t pskel_tuner_set_default_model(pskel_mlp_t* model);
+/* Device time per sweep of `sweeps` sweeps of candidate c on grid g (median of
+ * reps runs after one warm-up; the grid state advances). */
+pskel_status_t pskel_tuner_measure(pskel_grid_t* g, const pskel_stencil_desc_t* d, size_t sweeps,
+                                   const pskel_launch_cfg_t* c, int32_t reps, double* ms_per_sweep);
+
+/* ---- cost model: a small MLP trained and evaluated on the device by the
+ *      library's own kernels (forward, backward, Adam).  Inputs are
+ *      standardised per feature and the target by (y_mean, y_std), both part
+ *      of the model.  Parameters: one f32 vector in layer order
+ *      [W_1 (out x in, row-major), b_1, W_2, b_2, ...]; tanh hidden layers,
+ *      one linear output.  No counterpart in the reference, which searches
+ *      with GAlib (include/PSkelStencil.hpp:1865-2005). */
+pskel_status_t pskel_mlp_create(const int32_t* sizes, int32_t nlayers, uint64_t seed, int32_t device,
+                                pskel_mlp_t** out);
+pskel_status_t pskel_mlp_destroy(pskel_mlp_t* m);
+pskel_status_t pskel_mlp_num_params(const pskel_mlp_t* m, size_t* n);
+pskel_status_t pskel_mlp_get_params(const pskel_mlp_t* m, float* params, size_t n);
+pskel_status_t pskel_mlp_set_params(pskel_mlp_t* m, const float* params, size_t n);
+pskel_status_t pskel_mlp_set_normalization(pskel_mlp_t* m, const float* mean, const float* std,
+                                           float y_mean, float y_std);
+/* One forward + backward pass: mean squared error of the standardised target
+ * and its gradient with respect to the parameters. */
+pskel_status_t pskel_mlp_loss_grad(pskel_mlp_t* m, const float* x, const float* y, size_t n,
+                                   float* loss, float* grads);
+/* Minibatch Adam; final_loss is the mean batch loss of the last epoch. */
+pskel_status_t pskel_mlp_train(pskel_mlp_t* m, const float* x, const float* y, size_t n,
+                               int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                               float* final_loss);
+pskel_status_t pskel_mlp_predict(pskel_mlp_t* m, const float* x, size_t n, float* y);
+pskel_status_t pskel_mlp_save(const pskel_mlp_t* m, const char* path);
+pskel_status_t pskel_mlp_load(const char* path, int32_t device, pskel_mlp_t** out);
 
 /* ---- utilities ------------------------------------------------------------ */
 /* Plain device-to-device copy kernel (the bandwidth yardstick for roofline

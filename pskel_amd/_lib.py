@@ -113,6 +113,20 @@ class GridInfo(C.Structure):
     ]
 
 
+class TuneQuery(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("storage", C.c_int32),
+        ("width", C.c_size_t),
+        ("height", C.c_size_t),
+        ("depth", C.c_size_t),
+        ("iterations", C.c_size_t),
+    ]
+
+
+TUNER_MAX_CANDIDATES = 256
+TUNER_NFEAT = 24
+
 STRUCTS = [StencilDesc, LaunchCfg, RunStats, GridDesc, GridInfo]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpskel_amd.so")
@@ -151,6 +165,25 @@ def _declare(lib):
     _sig(lib, "pskel_grid_info", st, P, C.POINTER(GridInfo))
     _sig(lib, "pskel_run_iterative", st, P, C.POINTER(StencilDesc), SZ, C.POINTER(LaunchCfg), C.POINTER(RunStats))
     _sig(lib, "pskel_grid_residual", st, P, C.POINTER(D))
+    _sig(lib, "pskel_grid_swap", st, P)
+    F = C.POINTER(C.c_float)
+    _sig(lib, "pskel_tuner_candidates", st, C.POINTER(TuneQuery), C.POINTER(StencilDesc), C.POINTER(LaunchCfg), SZ,
+         C.POINTER(SZ))
+    _sig(lib, "pskel_tuner_features", st, C.POINTER(TuneQuery), C.POINTER(StencilDesc), C.POINTER(LaunchCfg), F)
+    _sig(lib, "pskel_tuner_choose", st, P, C.POINTER(TuneQuery), C.POINTER(StencilDesc), C.POINTER(LaunchCfg))
+    _sig(lib, "pskel_tuner_set_default_model", st, P)
+    _sig(lib, "pskel_tuner_measure", st, P, C.POINTER(StencilDesc), SZ, C.POINTER(LaunchCfg), I32, C.POINTER(D))
+    _sig(lib, "pskel_mlp_create", st, C.POINTER(I32), I32, C.c_uint64, I32, C.POINTER(P))
+    _sig(lib, "pskel_mlp_destroy", st, P)
+    _sig(lib, "pskel_mlp_num_params", st, P, C.POINTER(SZ))
+    _sig(lib, "pskel_mlp_get_params", st, P, F, SZ)
+    _sig(lib, "pskel_mlp_set_params", st, P, F, SZ)
+    _sig(lib, "pskel_mlp_set_normalization", st, P, F, F, C.c_float, C.c_float)
+    _sig(lib, "pskel_mlp_loss_grad", st, P, F, F, SZ, F, F)
+    _sig(lib, "pskel_mlp_train", st, P, F, F, SZ, I32, I32, C.c_float, C.c_uint64, F)
+    _sig(lib, "pskel_mlp_predict", st, P, F, SZ, F)
+    _sig(lib, "pskel_mlp_save", st, P, C.c_char_p)
+    _sig(lib, "pskel_mlp_load", st, C.c_char_p, I32, C.POINTER(P))
     _sig(lib, "pskel_device_copy", st, P, P, SZ, P, C.POINTER(D))
     _sig(lib, "pskel_device_alloc", st, SZ, C.POINTER(P))
     _sig(lib, "pskel_device_free", st, P)

@@ -28,9 +28,20 @@ def build_oracle(force: bool = True) -> str:
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
+def build_dropin_drivers(force: bool = True) -> str:
+    """The C++ programs that exercise the drop-in headers (tests/cpp)."""
+    d = os.path.join(ROOT, "tests", "cpp")
+    cmd = ["make", "-C", d]
+    if force:
+        cmd.insert(1, "-B")
+    subprocess.run(cmd, check=True)
+    return os.path.join(d, "bin")
+
+
 def main(argv=None):
     build_library()
     build_oracle()
+    build_dropin_drivers()
     return 0
 
 

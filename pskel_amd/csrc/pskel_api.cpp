@@ -821,6 +821,15 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
     return PSKEL_OK;
 }
 
+pskel_status_t pskel_grid_swap(pskel_grid_t* g)
+{
+    pskel_status_t st = check_grid(g);
+    if (st != PSKEL_OK) return st;
+    std::swap(g->buf[0], g->buf[1]);
+    g->parity ^= 1;
+    return PSKEL_OK;
+}
+
 pskel_status_t pskel_grid_residual(pskel_grid_t* g, double* out)
 {
     pskel_status_t st = check_grid(g);

@@ -1,0 +1,205 @@
+// Launch tuner: candidate launch shapes per kernel family, the cost model's
+// feature map, measurement, and the choice (learned model or heuristic).
+//
+// Replaces runIterativeAutoGPU's GAlib search (include/PSkelStencil.hpp:
+// 1865-2076), which maximised an analytic objective without timing anything.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pskel_internal.hpp"
+
+using namespace pskel;
+
+namespace pskel {
+pskel_status_t mlp_predict_host(const pskel_mlp_t* m, const float* x, size_t n, float* y);
+}
+
+namespace {
+
+pskel_mlp_t* g_default_model = nullptr;
+
+pskel_launch_cfg_t tuned(int tx, int ty, int tz, int k)
+{
+    pskel_launch_cfg_t c{};
+    c.variant = PSKEL_VARIANT_TUNED;
+    c.tile_x = tx;
+    c.tile_y = ty;
+    c.tile_z = tz;
+    c.time_tile = k;
+    return c;
+}
+
+pskel_launch_cfg_t plain(int bx, int by)
+{
+    pskel_launch_cfg_t c{};
+    c.variant = PSKEL_VARIANT_PLAIN;
+    c.block_x = bx;
+    c.block_y = by;
+    return c;
+}
+
+std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const pskel_stencil_desc_t& d)
+{
+    std::vector<pskel_launch_cfg_t> v;
+    const int pat = classify(d, q.ndim);
+    if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9 || pat == PAT_LIFE8)) {
+        const int ry_nwy[][2] = {{1, 2}, {1, 4}, {2, 2}, {2, 4}, {4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}};
+        for (auto& t : ry_nwy) v.push_back(tuned(0, t[0], t[1], 1));
+        for (int k = 2; k <= 4; ++k)
+            for (int cy : {16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
+        v.push_back(plain(64, 4));
+    } else if (q.ndim == 3 && pat == PAT_J7) {
+        const bool narrow = q.storage != PSKEL_DTYPE_F32;
+        const int vds16[] = {2, 4, 8}, vds32[] = {1, 2, 4};
+        for (int vd : (narrow ? vds16 : vds32))
+            for (int rw : {1, 2, 4})
+                for (int hs : {16, 32, 64, 128, 256}) {
+                    const int lds = 2 * (4 * rw + 2) * (64 * vd + 2 * vd) * 4;
+                    if (lds > 64 * 1024) continue;
+                    v.push_back(tuned(vd, rw, hs, 1));
+                }
+        v.push_back(plain(64, 4));
+    } else {
+        const int blocks[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 2}, {256, 1}, {32, 4}};
+        for (auto& b : blocks) v.push_back(plain(b[0], b[1]));
+    }
+    return v;
+}
+
+float lg2(double x) { return (float)std::log2(std::max(x, 1.0)); }
+
+void features(const pskel_tune_query_t& q, const pskel_stencil_desc_t& d, const pskel_launch_cfg_t& c, float* f)
+{
+    std::memset(f, 0, sizeof(float) * PSKEL_TUNER_NFEAT);
+    const int pat = classify(d, q.ndim);
+    int i = 0;
+    f[i++] = lg2((double)q.width);
+    f[i++] = lg2((double)q.height);
+    f[i++] = lg2((double)(q.ndim == 3 ? q.depth : 1));
+    f[i++] = lg2((double)q.width * q.height * (q.ndim == 3 ? q.depth : 1));
+    f[i++] = (float)desc_radius(d, q.ndim);
+    f[i++] = q.storage == PSKEL_DTYPE_U8 ? 1.f : 0.f;
+    f[i++] = q.storage == PSKEL_DTYPE_F32 ? 1.f : 0.f;
+    f[i++] = (q.storage == PSKEL_DTYPE_BF16 || q.storage == PSKEL_DTYPE_F16) ? 1.f : 0.f;
+    f[i++] = lg2((double)q.iterations);
+    f[i++] = pat == PAT_J5 ? 1.f : 0.f;
+    f[i++] = pat == PAT_BOX9 ? 1.f : 0.f;
+    f[i++] = pat == PAT_LIFE8 ? 1.f : 0.f;
+    f[i++] = pat == PAT_J7 ? 1.f : 0.f;
+    f[i++] = (float)d.npoints;
+    // candidate
+    f[i++] = c.variant == PSKEL_VARIANT_TUNED ? 1.f : 0.f;
+    f[i++] = lg2((double)std::max(c.tile_x, 1));
+    f[i++] = lg2((double)std::max(c.tile_y, 1));
+    f[i++] = lg2((double)std::max(c.tile_z, 1));
+    f[i++] = (float)std::max(c.time_tile, 1);
+    f[i++] = lg2((double)std::max(c.block_x, 1));
+    f[i++] = lg2((double)std::max(c.block_y, 1));
+    // interactions the model would otherwise have to discover
+    f[i++] = (float)std::max(c.time_tile, 1) * (q.ndim == 2 ? 1.f : 0.f) / (float)std::max(c.tile_y, 1);
+    f[i++] = lg2((double)q.width * q.height * (q.ndim == 3 ? q.depth : 1) /
+                 (double)std::max(1, c.tile_z > 0 && q.ndim == 3 ? c.tile_z : 1));
+    f[i++] = 1.f;  // bias-like constant
+}
+
+// Heuristic: the shapes the sweeps on MI355X found best for the large grids.
+pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_desc_t& d)
+{
+    const int pat = classify(d, q.ndim);
+    if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9 || pat == PAT_LIFE8)) {
+        const double cells = (double)q.width * q.height;
+        if (q.iterations >= 4 && cells >= (1 << 20)) return tuned(0, 16, 0, pat == PAT_BOX9 ? 3 : 4);
+        return tuned(0, 2, 4, 1);
+    }
+    if (q.ndim == 3 && pat == PAT_J7) {
+        if (q.storage == PSKEL_DTYPE_F32) return tuned(4, 1, 128, 1);
+        return tuned(4, 1, q.storage == PSKEL_DTYPE_BF16 ? 32 : 64, 1);
+    }
+    return plain(64, 4);
+}
+
+pskel_status_t check_query(const pskel_tune_query_t* q, const pskel_stencil_desc_t* d)
+{
+    if (!q || !d) return fail(PSKEL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (q->ndim != 2 && q->ndim != 3) return fail(PSKEL_ERR_INVALID_ARGUMENT, "ndim must be 2 or 3");
+    if (d->npoints < 1 || d->npoints > PSKEL_MAX_POINTS) return fail(PSKEL_ERR_INVALID_ARGUMENT, "npoints");
+    return PSKEL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pskel_status_t pskel_tuner_candidates(const pskel_tune_query_t* q, const pskel_stencil_desc_t* d,
+                                      pskel_launch_cfg_t* out, size_t cap, size_t* n)
+{
+    pskel_status_t st = check_query(q, d);
+    if (st != PSKEL_OK) return st;
+    const auto v = candidates(*q, *d);
+    if (n) *n = v.size();
+    if (out)
+        for (size_t i = 0; i < v.size() && i < cap; ++i) out[i] = v[i];
+    return PSKEL_OK;
+}
+
+pskel_status_t pskel_tuner_features(const pskel_tune_query_t* q, const pskel_stencil_desc_t* d,
+                                    const pskel_launch_cfg_t* c, float* feat)
+{
+    pskel_status_t st = check_query(q, d);
+    if (st != PSKEL_OK) return st;
+    if (!c || !feat) return fail(PSKEL_ERR_INVALID_ARGUMENT, "NULL argument");
+    features(*q, *d, *c, feat);
+    return PSKEL_OK;
+}
+
+pskel_status_t pskel_tuner_set_default_model(pskel_mlp_t* model)
+{
+    g_default_model = model;
+    return PSKEL_OK;
+}
+
+pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_query_t* q,
+                                  const pskel_stencil_desc_t* d, pskel_launch_cfg_t* out)
+{
+    pskel_status_t st = check_query(q, d);
+    if (st != PSKEL_OK) return st;
+    if (!out) return fail(PSKEL_ERR_INVALID_ARGUMENT, "out is NULL");
+    if (!model) model = g_default_model;
+    if (!model) {
+        *out = heuristic(*q, *d);
+        return PSKEL_OK;
+    }
+    const auto v = candidates(*q, *d);
+    std::vector<float> x(v.size() * PSKEL_TUNER_NFEAT), y(v.size());
+    for (size_t i = 0; i < v.size(); ++i) features(*q, *d, v[i], &x[i * PSKEL_TUNER_NFEAT]);
+    st = mlp_predict_host(model, x.data(), v.size(), y.data());
+    if (st != PSKEL_OK) return st;
+    const size_t best = (size_t)(std::min_element(y.begin(), y.end()) - y.begin());
+    *out = v[best];
+    return PSKEL_OK;
+}
+
+pskel_status_t pskel_tuner_measure(pskel_grid_t* g, const pskel_stencil_desc_t* d, size_t sweeps,
+                                   const pskel_launch_cfg_t* c, int32_t reps, double* ms_per_sweep)
+{
+    if (!g || !d || !c || !ms_per_sweep || sweeps == 0) return fail(PSKEL_ERR_INVALID_ARGUMENT, "bad argument");
+    reps = std::max(reps, 1);
+    pskel_run_stats_t st{};
+    pskel_status_t s = pskel_run_iterative(g, d, sweeps, c, &st);  // warm-up
+    if (s != PSKEL_OK) return s;
+    std::vector<double> t;
+    for (int r = 0; r < reps; ++r) {
+        s = pskel_run_iterative(g, d, sweeps, c, &st);
+        if (s != PSKEL_OK) return s;
+        t.push_back(st.elapsed_ms / (double)sweeps);
+    }
+    std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
+    *ms_per_sweep = t[t.size() / 2];
+    return PSKEL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+// Driver for the PSkel drop-in headers, used by tests/test_dropin_*.py.
+//
+//   drive <kernel> <method> <W> <H> <D> <iters> <input.bin> <output0.bin> <result.bin>
+//
+// kernel: jacobi2d | diffusion9 | life | jacobi3d | user2d | user3d
+// method: cpu | seq | gpu | pyramid | tiling | auto
+// Arrays are raw little-endian host arrays in Array2D/Array3D order
+// (float, or uint8 for life).  The program is written the way a PSkel user
+// writes one: Arrays, a Mask, an Args, Stencil2D/3D, one run method.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "PSkel.h"
+
+using namespace PSkel;
+
+// A user kernel with no descriptor: the GPU methods run it on the device only
+// when this file is compiled by hipcc.
+struct UserJacobiArgs {
+    float c;
+};
+
+namespace PSkel {
+__parallel__ void stencilKernel(Array2D<float> in, Array2D<float> out, Mask2D<float> mask, UserJacobiArgs a,
+                                size_t h, size_t w)
+{
+    (void)mask;
+    out(h, w) = a.c * (((in(h - 1, w) + in(h + 1, w)) + in(h, w - 1)) + in(h, w + 1));
+}
+
+__parallel__ void stencilKernel(Array3D<float> in, Array3D<float> out, Mask3D<float> mask, UserJacobiArgs a,
+                                size_t h, size_t w, size_t d)
+{
+    (void)mask;
+    if (h == 0 || w == 0 || d == 0 || h + 1 == in.getHeight() || w + 1 == in.getWidth() || d + 1 == in.getDepth()) {
+        out(h, w, d) = in(h, w, d);
+        return;
+    }
+    float acc = in(h - 1, w, d) + in(h + 1, w, d);
+    acc = acc + in(h, w - 1, d);
+    acc = acc + in(h, w + 1, d);
+    acc = acc + in(h, w, d - 1);
+    acc = acc + in(h, w, d + 1);
+    out(h, w, d) = a.c * acc;
+}
+}  // namespace PSkel
+
+template <typename T>
+static bool readf(const char* p, T* dst, size_t n)
+{
+    FILE* f = std::fopen(p, "rb");
+    if (!f) return false;
+    const bool ok = std::fread(dst, sizeof(T), n, f) == n;
+    std::fclose(f);
+    return ok;
+}
+
+template <typename T>
+static bool writef(const char* p, const T* src, size_t n)
+{
+    FILE* f = std::fopen(p, "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(src, sizeof(T), n, f) == n;
+    std::fclose(f);
+    return ok;
+}
+
+template <class St>
+static int runMethod(St& st, const std::string& method, size_t iters)
+{
+    if (method == "cpu") st.runIterativeCPU(iters, 0);
+    else if (method == "seq") st.runIterativeSequential(iters);
+#ifdef PSKEL_GPU
+    else if (method == "gpu") st.runIterativeGPU(iters, 32, 4);
+    else if (method == "pyramid") st.runIterativeGPU(iters, 2, 32, 4);
+    else if (method == "tiling") st.runIterativeTilingGPU(iters, 256, 256, 1, 3);
+    else if (method == "auto") st.runIterativeAutoGPU(iters);
+#endif
+    else {
+        std::fprintf(stderr, "unknown method %s\n", method.c_str());
+        return 2;
+    }
+    return 0;
+}
+
+template <typename T, class Mask, class Args>
+static int run2d(Mask mask, Args args, const std::string& method, size_t W, size_t H, size_t iters, char** files)
+{
+    Array2D<T> in(W, H), out(W, H);
+    if (!readf(files[0], in.hostArray, W * H) || !readf(files[1], out.hostArray, W * H)) return 3;
+    Stencil2D<Array2D<T>, Mask, Args> st(in, out, mask, args);
+    const int rc = runMethod(st, method, iters);
+    if (rc) return rc;
+    if (!writef(files[2], out.hostArray, W * H)) return 4;
+    in.hostFree();
+    out.hostFree();
+    return 0;
+}
+
+template <class Mask, class Args>
+static int run3d(Mask mask, Args args, const std::string& method, size_t W, size_t H, size_t D, size_t iters,
+                 char** files)
+{
+    Array3D<float> in(W, H, D), out(W, H, D);
+    const size_t n = W * H * D;
+    if (!readf(files[0], in.hostArray, n) || !readf(files[1], out.hostArray, n)) return 3;
+    Stencil3D<Array3D<float>, Mask, Args> st(in, out, mask, args);
+    const int rc = runMethod(st, method, iters);
+    if (rc) return rc;
+    if (!writef(files[2], out.hostArray, n)) return 4;
+    in.hostFree();
+    out.hostFree();
+    return 0;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 10) {
+        std::fprintf(stderr, "usage: drive kernel method W H D iters input output0 result\n");
+        return 2;
+    }
+    const std::string kernel = argv[1], method = argv[2];
+    const size_t W = std::strtoull(argv[3], nullptr, 10), H = std::strtoull(argv[4], nullptr, 10),
+                 D = std::strtoull(argv[5], nullptr, 10), iters = std::strtoull(argv[6], nullptr, 10);
+    char** files = argv + 7;
+    try {
+        if (kernel == "jacobi2d") return run2d<float>(jacobiMask2D<float>(), ScaledSumArgs{0.25, 0}, method, W, H, iters, files);
+        if (kernel == "user2d") return run2d<float>(jacobiMask2D<float>(), UserJacobiArgs{0.25f}, method, W, H, iters, files);
+        if (kernel == "diffusion9") {
+            // centre 0.5, edge 0.1, corner 0.025, skewed by the wind (0.02, -0.01)
+            const double wx = 0.02, wy = -0.01;
+            float wts[9];
+            for (int i = 0; i < 9; ++i) {
+                const int dh = i / 3 - 1, dw = i % 3 - 1;
+                double base, g;
+                if (!dh && !dw) base = 0.5, g = 0.0;
+                else if (!dh || !dw) base = 0.1, g = 1.0;
+                else base = 0.025, g = 0.25;
+                wts[i] = (float)(base - g * (dw * wx + dh * wy));
+            }
+            return run2d<float>(boxMask2D<float>(wts), WeightedSumArgs{0}, method, W, H, iters, files);
+        }
+        if (kernel == "life") return run2d<unsigned char>(mooreMask2D<unsigned char>(), LifeArgs{0}, method, W, H, iters, files);
+        if (kernel == "jacobi3d") return run3d(jacobiMask3D<float>(), ScaledSumArgs{1.0 / 6.0, BOUNDARY_COPY}, method, W, H, D, iters, files);
+        if (kernel == "user3d") return run3d(jacobiMask3D<float>(), UserJacobiArgs{(float)(1.0 / 6.0)}, method, W, H, D, iters, files);
+    } catch (const PSkel::Error& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 10 + e.status;
+    }
+    std::fprintf(stderr, "unknown kernel %s\n", kernel.c_str());
+    return 2;
+}
