@@ -121,6 +121,92 @@ def measure(name, variant, steps, warmup, field=None, run_kw=None):
     }
 
 
+def batch_fields(seed=5):
+    """The 256-grid workload: sides log-uniform in [64, 2048] per axis, half
+    Jacobi f32 (interior uniform [0,1), zero ring), half life u8 (p = 0.3)."""
+    from pskel_amd import inputs
+
+    shapes = inputs.batch_shapes(256, seed=seed)
+    fields = []
+    for i, (h, w, kind) in enumerate(shapes):
+        if kind == "life":
+            fields.append((inputs.uniform(1000 + i, 0, h * w).reshape(h, w) < np.float32(0.3)).astype(np.uint8))
+        else:
+            fields.append(inputs.jacobi2d_field(h, w, seed=1000 + i))
+    return shapes, fields
+
+
+def batch_rows(shapes, sweeps):
+    """Rows per lane per grid from the cost model (restricted to the batch
+    kernel's shapes: 4 waves, 1/2/4/8 rows), 4 when no model is installed."""
+    from pskel_amd import tuner as T
+
+    model = T.default_model()
+    if model is None:
+        return [4] * len(shapes), "default (4 rows per lane)"
+    rows = []
+    for h, w, kind in shapes:
+        q = T.Query((h, w), "u8" if kind == "life" else "f32", "life" if kind == "life" else "jacobi2d", sweeps)
+        cands = [c for c in T.candidates(q) if c.variant == 1 and c.time_tile <= 1 and c.tile_z == 4
+                 and c.tile_y in (1, 2, 4, 8)]
+        pred = model.predict(T.features(q, cands))
+        rows.append(int(cands[int(np.argmin(pred))].tile_y))
+    return rows, "cost model"
+
+
+def measure_batch(steps, warmup, sweeps=200):
+    """256 uneven grids, `sweeps` sweeps per step: one persistent kernel per
+    sweep, captured in one hipGraph; beside it the same work launched grid by
+    grid (one runIterativeGPU per grid, the reference's way)."""
+    from pskel_amd import api
+    from pskel_amd import kernels as K
+
+    shapes, fields = batch_fields()
+    grids, stencils = [], []
+    for (h, w, kind), f in zip(shapes, fields):
+        g = api.Grid((h, w), "u8" if kind == "life" else "f32")
+        g.upload_pair(f)
+        grids.append(g)
+        stencils.append(K.life() if kind == "life" else K.jacobi2d())
+    rows, how = batch_rows(shapes, sweeps)
+    cells = sum(h * w for h, w, _ in shapes)
+    nbytes = sum(h * w * (1 if k == "life" else 4) * 2 for h, w, k in shapes)
+    try:
+        with api.Batch(grids, stencils, rows) as b:
+            for _ in range(warmup):
+                b.run(sweeps, use_graph=True)
+            api.synchronize()
+            t0 = time.perf_counter()
+            dev = 0.0
+            for _ in range(steps):
+                dev += b.run(sweeps, use_graph=True)["elapsed_ms"]
+            api.synchronize()
+            wall = time.perf_counter() - t0
+            info = b.info()
+        # separate launches: every grid on its own, tuned kernel, no graph
+        t1 = time.perf_counter()
+        for g, s in zip(grids, stencils):
+            g.run(s, sweeps, variant="tuned")
+        api.synchronize()
+        sep = time.perf_counter() - t1
+    finally:
+        for g in grids:
+            g.close()
+    return {
+        "workload": f"batch 256 grids (sides 64..2048) jacobi f32 + life u8 x{sweeps}, one hipGraph",
+        "cell_updates_per_s": cells * sweeps * steps / wall,
+        "ms_per_step": wall * 1e3 / steps,
+        "device_ms_per_step": dev / steps,
+        "hbm_roofline_frac": nbytes * sweeps * steps / wall / (HBM_PEAK_GBS * 1e9),
+        "separate_launches_ms": sep * 1e3,
+        "speedup_vs_separate": sep / (wall / steps),
+        "items": info["items"],
+        "persistent_blocks": info["blocks"],
+        "rows_per_lane_from": how,
+        "cells": cells,
+    }
+
+
 def cpu_baseline(name, seconds_target=12.0):
     """The oracle (runIterativeCPU restatement, OpenMP) on a bounded sample:
     the full grid for as many sweeps as fit about `seconds_target`."""
@@ -233,7 +319,8 @@ def main(argv=None):
     extra = {}
     if not args.no_extra:
         plan = [("jacobi3d", "plain"), ("jacobi3d_bf16", "tuned"), ("jacobi3d_f16", "tuned"),
-                ("life", "tuned"), ("life", "plain")]
+                ("life", "tuned"), ("life", "plain"), ("jacobi2d", "tuned"), ("jacobi2d", "plain"),
+                ("diffusion9", "tuned"), ("diffusion9", "plain")]
         for nm, var in plan:
             if (nm, var) == (args.workload, args.variant):
                 continue
@@ -242,6 +329,10 @@ def main(argv=None):
                                                        "avg_launch_ms", "hbm_roofline_frac", "launch")}
             print(f"# {nm} {var}: {r['cell_updates_per_s']:.4g} CU/s {r['hbm_roofline_frac']*100:.1f}%",
                   file=sys.stderr, flush=True)
+        rb = measure_batch(max(1, min(args.steps, 3)), 1)
+        extra["batch256:graph"] = rb
+        print(f"# batch256: {rb['cell_updates_per_s']:.4g} CU/s, {rb['ms_per_step']:.2f} ms/step, "
+              f"{rb['speedup_vs_separate']:.1f}x vs separate launches", file=sys.stderr, flush=True)
 
     bytes_per_launch = head["algorithmic_bytes_per_cu"] * head["cells"] * head["sweeps_per_launch"]
     achieved = bytes_per_launch / (head["avg_launch_ms"] * 1e-3) / 1e9

@@ -195,6 +195,23 @@ pskel_status_t pskel_grid_swap(pskel_grid_t* grid);
  * reference leaves to the caller). */
 pskel_status_t pskel_grid_residual(pskel_grid_t* grid, double* out);
 
+/* ---- batched persistent sweeps: many uneven 2D grids advanced together by
+ *      one persistent kernel per sweep (a fixed set of blocks walking a list
+ *      of (grid, tile) work items), the sweep sequence captured in one
+ *      hipGraph.  The reference runs one Stencil at a time with a launch and a
+ *      host sync per sweep (include/PSkelStencil.hpp:1078-1082, 1790-1791).
+ *      Grids: 2D, one device, Jacobi 5-point or 3x3 weighted (F32) or life
+ *      (U8); cfgs[i].tile_y picks the rows per lane of grid i (1, 2, 4, 8). */
+typedef struct pskel_batch pskel_batch_t;
+pskel_status_t pskel_batch_create(pskel_grid_t* const* grids, const pskel_stencil_desc_t* descs,
+                                  const pskel_launch_cfg_t* cfgs, size_t n, pskel_batch_t** out);
+/* `iterations` sweeps of every grid; use_graph: replay a captured hipGraph
+ * (captured on first use for this iteration count). */
+pskel_status_t pskel_batch_run(pskel_batch_t* b, size_t iterations, int32_t use_graph,
+                               pskel_run_stats_t* stats);
+pskel_status_t pskel_batch_info(const pskel_batch_t* b, int32_t* nitems, int32_t* nblocks);
+pskel_status_t pskel_batch_destroy(pskel_batch_t* b);
+
 /* ---- launch tuner (replaces runIterativeAutoGPU's GAlib search over an
  *      analytic objective, include/PSkelStencil.hpp:1865-2076).  Candidates
  *      are the launch configurations of the kernel family a descriptor maps
@@ -220,7 +237,9 @@ pskel_status_t pskel_tuner_candidates(const pskel_tune_query_t* q, const pskel_s
 pskel_status_t pskel_tuner_features(const pskel_tune_query_t* q, const pskel_stencil_desc_t* d,
                                     const pskel_launch_cfg_t* c, float* feat);
 /* The candidate with the lowest predicted time.  model NULL: the model installed
- * with pskel_tuner_set_default_model, else a fixed heuristic per kernel family. */
+ * with pskel_tuner_set_default_model, else the model shipped next to the
+ * library (<libdir>/data/costmodel.bin, loaded on first use), else a fixed
+ * heuristic per kernel family. */
 pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_query_t* q,
                                   const pskel_stencil_desc_t* d, pskel_launch_cfg_t* out);
 pskel_status_t pskel_tuner_set_default_model(pskel_mlp_t* model);

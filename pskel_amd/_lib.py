@@ -167,6 +167,10 @@ def _declare(lib):
     _sig(lib, "pskel_grid_residual", st, P, C.POINTER(D))
     _sig(lib, "pskel_grid_swap", st, P)
     F = C.POINTER(C.c_float)
+    _sig(lib, "pskel_batch_create", st, C.POINTER(P), C.POINTER(StencilDesc), C.POINTER(LaunchCfg), SZ, C.POINTER(P))
+    _sig(lib, "pskel_batch_run", st, P, SZ, I32, C.POINTER(RunStats))
+    _sig(lib, "pskel_batch_info", st, P, C.POINTER(I32), C.POINTER(I32))
+    _sig(lib, "pskel_batch_destroy", st, P)
     _sig(lib, "pskel_tuner_candidates", st, C.POINTER(TuneQuery), C.POINTER(StencilDesc), C.POINTER(LaunchCfg), SZ,
          C.POINTER(SZ))
     _sig(lib, "pskel_tuner_features", st, C.POINTER(TuneQuery), C.POINTER(StencilDesc), C.POINTER(LaunchCfg), F)

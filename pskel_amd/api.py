@@ -147,6 +147,52 @@ class Grid:
         return v.value
 
 
+class Batch:
+    """Many 2D grids swept together by one persistent kernel per sweep
+    (pskel_batch_t).  grids: list of Grid; stencils: one per grid; rows: rows
+    per lane per grid (1, 2, 4 or 8)."""
+
+    def __init__(self, grids, stencils, rows=None):
+        n = len(grids)
+        self.grids = list(grids)
+        hs = (C.c_void_p * n)(*[g._h for g in grids])
+        ds = (L.StencilDesc * n)(*[s.desc() for s in stencils])
+        cs = (L.LaunchCfg * n)()
+        for i in range(n):
+            cs[i].variant = L.VARIANT_TUNED
+            cs[i].tile_y = int(rows[i]) if rows is not None else 4
+        h = C.c_void_p()
+        L.check(L.lib().pskel_batch_create(hs, ds, cs, n, C.byref(h)))
+        self._h = h
+
+    def run(self, iterations: int, use_graph: bool = True) -> Dict:
+        st = L.RunStats()
+        L.check(L.lib().pskel_batch_run(self._h, int(iterations), 1 if use_graph else 0, C.byref(st)))
+        return {"elapsed_ms": st.elapsed_ms, "launches": st.launches, "sweeps": st.sweeps}
+
+    def info(self) -> Dict:
+        a, b = C.c_int32(), C.c_int32()
+        L.check(L.lib().pskel_batch_info(self._h, C.byref(a), C.byref(b)))
+        return {"items": a.value, "blocks": b.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().pskel_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def make_cfg(variant="plain", block=(0, 0), tile=(0, 0, 0), time_tile=1, residual_every=0,
              use_graph=False, stream=0) -> L.LaunchCfg:
     cfg = L.LaunchCfg()

@@ -11,7 +11,7 @@
 //              exchanged across lanes, all row loads issued before any math.
 //   k2d_ttile  overlapped temporal tile: K sweeps in LDS per launch, K-cell halo
 //              recomputed redundantly (the rewrite of PSkelStencilTiling).
-#include "k_common.hpp"
+#include "k_tiles2d.hpp"
 
 namespace pskel {
 
@@ -66,105 +66,10 @@ hipError_t plain_dispatch(const Sweep2D& s, const Points& p, dim3 grid, dim3 blo
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// point evaluation over a register window (radius 1 patterns)
-// ---------------------------------------------------------------------------
-// Window access: row i (0..2 = h-1..h+1), column j in -1..4 relative to the
-// lane's float4 (-1: left halo, 4: right halo).
-struct Win {
-    const float4* v;  // 3 rows
-    const float* lf;  // 3 left halos
-    const float* rt;  // 3 right halos
-    __device__ __forceinline__ float at(int i, int j) const
-    {
-        if (j < 0) return lf[i];
-        if (j > 3) return rt[i];
-        const float4 x = v[i];
-        return j == 0 ? x.x : j == 1 ? x.y : j == 2 ? x.z : x.w;
-    }
-};
-
-template <int PAT>
-__device__ __forceinline__ float eval_pat(const Win& W, int j, const Points& p)
-{
-    if constexpr (PAT == PAT_J5) {
-        float acc = W.at(0, j) + W.at(2, j);
-        acc = acc + W.at(1, j - 1);
-        acc = acc + W.at(1, j + 1);
-        return p.coeff * acc;
-    } else {  // PAT_BOX9, row-major weighted sum
-        float acc = p.wt[0] * W.at(0, j - 1);
-        acc = acc + p.wt[1] * W.at(0, j);
-        acc = acc + p.wt[2] * W.at(0, j + 1);
-        acc = acc + p.wt[3] * W.at(1, j - 1);
-        acc = acc + p.wt[4] * W.at(1, j);
-        acc = acc + p.wt[5] * W.at(1, j + 1);
-        acc = acc + p.wt[6] * W.at(2, j - 1);
-        acc = acc + p.wt[7] * W.at(2, j);
-        acc = acc + p.wt[8] * W.at(2, j + 1);
-        return acc;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// tuned: 64 lanes x float4 = 256 columns per wave, RY rows per lane.
-// block = (64, NWY): the NWY waves stack along h.
-// ---------------------------------------------------------------------------
 template <int PAT, int RY>
 __global__ __launch_bounds__(256) void k2d_tuned(Sweep2D s, Points p)
 {
-    const int lane = threadIdx.x;
-    const int w0 = blockIdx.x * 256;
-    const int w = w0 + lane * 4;
-    const int hb = (blockIdx.y * blockDim.y + threadIdx.y) * RY;  // first output row
-    const int H = s.H, Wd = s.W, r = s.radius;
-    const float* __restrict__ src = (const float*)s.src;
-    float* __restrict__ dst = (float*)s.dst;
-    const size_t pitch = s.pitch;
-    const bool col_live = w < Wd;
-
-    float4 v[RY + 2];
-    float lfl[RY + 2], rtl[RY + 2];
-#pragma unroll
-    for (int i = 0; i < RY + 2; ++i) {
-        const int h = hb - 1 + i;
-        v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        lfl[i] = 0.f;
-        rtl[i] = 0.f;
-        if (h >= 0 && h < H) {
-            const float* row = src + (size_t)h * pitch;
-            if (col_live) v[i] = *reinterpret_cast<const float4*>(row + w);
-            if (lane == 0 && w0 > 0) lfl[i] = row[w0 - 1];
-            if (lane == kWave - 1 && w0 + 256 < Wd) rtl[i] = row[w0 + 256];
-        }
-    }
-    float lf[RY + 2], rt[RY + 2];
-#pragma unroll
-    for (int i = 0; i < RY + 2; ++i) {
-        const float up = __shfl_up(v[i].w, 1, kWave);
-        const float dn = __shfl_down(v[i].x, 1, kWave);
-        lf[i] = lane == 0 ? lfl[i] : up;
-        rt[i] = lane == kWave - 1 ? rtl[i] : dn;
-    }
-    if (!col_live) return;
-    const bool full = (w >= r) && (w + 3 < Wd - r);
-#pragma unroll
-    for (int i = 1; i <= RY; ++i) {
-        const int h = hb - 1 + i;
-        if (h < r || h >= H - r) continue;
-        const Win win{&v[i - 1], &lf[i - 1], &rt[i - 1]};
-        float o[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) o[j] = eval_pat<PAT>(win, j, p);
-        float* row = dst + (size_t)h * pitch + w;
-        if (full) {
-            *reinterpret_cast<float4*>(row) = make_float4(o[0], o[1], o[2], o[3]);
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (w + j >= r && w + j < Wd - r) row[j] = o[j];
-        }
-    }
+    tile_f32<PAT, RY>(s, p, blockIdx.x, blockIdx.y);
 }
 
 template <int PAT>

@@ -188,16 +188,6 @@ using namespace pskel;
 // ---------------------------------------------------------------------------
 // the opaque grid type of the C ABI
 // ---------------------------------------------------------------------------
-struct GraphEntry {
-    std::vector<unsigned char> key;
-    hipGraphExec_t exec = nullptr;
-    int64_t launches = 0;
-    int64_t residuals = 0;
-};
-
-struct pskel_grid : pskel::Grid {
-    std::vector<GraphEntry> graphs;
-};
 
 namespace {
 
@@ -329,10 +319,12 @@ size_t next_k(size_t done, size_t iterations, int kmax, size_t every)
     return k;
 }
 
+}  // namespace
+
 // Ping-pong state after a launch of k fused sweeps.  Odd k wrote the other
 // buffer: swap.  Even k wrote the spare (it cannot write the buffer it reads):
 // the spare becomes current and the old current becomes the spare.
-void advance_state(pskel::Grid& g, size_t k)
+void pskel::advance_state(pskel::Grid& g, size_t k)
 {
     if (k % 2) {
         std::swap(g.buf[0], g.buf[1]);
@@ -342,6 +334,8 @@ void advance_state(pskel::Grid& g, size_t k)
         g.spare_parity = g.parity;
     }
 }
+
+namespace {
 
 // Issue all launches for `iterations` sweeps; advances the grid state.
 pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Points& pts, size_t iterations,

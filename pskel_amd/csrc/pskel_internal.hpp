@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/pskel_amd.h"
 
@@ -89,6 +90,8 @@ struct Grid {
     size_t res_cap = 0;
 };
 
+void advance_state(Grid& g, size_t k);
+
 // ---------------------------------------------------------------------------
 // Launch plumbing shared by the kernel TUs.
 // ---------------------------------------------------------------------------
@@ -143,3 +146,15 @@ hipError_t launch_fill_zero(void* dst, size_t bytes, hipStream_t st);
 hipError_t launch_copy_ring2d(const Sweep2D& s, int elem, hipStream_t st);
 
 }  // namespace pskel
+
+// The opaque grid of the C ABI: the resident pair plus its captured graphs.
+struct GraphEntry {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+    int64_t launches = 0;
+    int64_t residuals = 0;
+};
+
+struct pskel_grid : pskel::Grid {
+    std::vector<GraphEntry> graphs;
+};

@@ -3,9 +3,12 @@
 //
 // Replaces runIterativeAutoGPU's GAlib search (include/PSkelStencil.hpp:
 // 1865-2076), which maximised an analytic objective without timing anything.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -21,6 +24,25 @@ pskel_status_t mlp_predict_host(const pskel_mlp_t* m, const float* x, size_t n, 
 namespace {
 
 pskel_mlp_t* g_default_model = nullptr;
+std::once_flag g_model_once;
+
+// The shipped cost model lives next to the library: <libdir>/data/costmodel.bin.
+void load_shipped_model()
+{
+    Dl_info info;
+    if (!dladdr((void*)&load_shipped_model, &info) || !info.dli_fname) return;
+    std::string dir(info.dli_fname);
+    const size_t slash = dir.rfind('/');
+    dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash);
+    const std::string path = dir + "/data/costmodel.bin";
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return;
+    std::fclose(f);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    pskel_mlp_t* m = nullptr;
+    if (pskel_mlp_load(path.c_str(), dev, &m) == PSKEL_OK) g_default_model = m;
+}
 
 pskel_launch_cfg_t tuned(int tx, int ty, int tz, int k)
 {
@@ -158,6 +180,7 @@ pskel_status_t pskel_tuner_features(const pskel_tune_query_t* q, const pskel_ste
 
 pskel_status_t pskel_tuner_set_default_model(pskel_mlp_t* model)
 {
+    std::call_once(g_model_once, [] {});  // an explicit choice wins over the shipped file
     g_default_model = model;
     return PSKEL_OK;
 }
@@ -168,7 +191,10 @@ pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_que
     pskel_status_t st = check_query(q, d);
     if (st != PSKEL_OK) return st;
     if (!out) return fail(PSKEL_ERR_INVALID_ARGUMENT, "out is NULL");
-    if (!model) model = g_default_model;
+    if (!model) {
+        std::call_once(g_model_once, load_shipped_model);
+        model = g_default_model;
+    }
     if (!model) {
         *out = heuristic(*q, *d);
         return PSKEL_OK;

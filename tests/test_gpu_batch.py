@@ -1,0 +1,96 @@
+"""Batched persistent sweeps of uneven grids: every grid bit-exact to the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from pskel_amd import api, inputs
+from pskel_amd import kernels as K
+from pskel_amd._lib import PSkelError
+
+pytestmark = pytest.mark.gpu
+
+
+def _make(shapes, seed=0):
+    grids, stencils, fields = [], [], []
+    for i, (h, w, kind) in enumerate(shapes):
+        if kind == "life":
+            f = (inputs.uniform(seed + i, 0, h * w).reshape(h, w) < 0.3).astype(np.uint8)
+            g = api.Grid((h, w), "u8")
+            st = K.life()
+        else:
+            f = inputs.jacobi2d_field(h, w, seed=seed + i)
+            g = api.Grid((h, w), "f32")
+            st = K.jacobi2d() if kind == "jacobi" else K.diffusion9()
+        g.upload_pair(f)
+        grids.append(g)
+        stencils.append(st)
+        fields.append(f)
+    return grids, stencils, fields
+
+
+SHAPES = [(64, 64, "jacobi"), (70, 1030, "life"), (257, 66, "jacobi"), (1000, 300, "life"),
+          (3, 9, "jacobi"), (129, 513, "diff9"), (2048, 64, "life"), (333, 777, "jacobi")]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("rows", [None, [1, 2, 4, 8, 1, 2, 4, 8]])
+def test_batch_matches_oracle(gpu, use_graph, rows):
+    grids, stencils, fields = _make(SHAPES)
+    try:
+        with api.Batch(grids, stencils, rows) as b:
+            info = b.info()
+            assert info["items"] >= len(grids) and 1 <= info["blocks"] <= info["items"]
+            b.run(7, use_graph=use_graph)
+            b.run(6, use_graph=use_graph)  # resume: 13 sweeps in total
+            for g, s, f in zip(grids, stencils, fields):
+                assert np.array_equal(g.download(), oracle.run_2d(s, f, 13)), g.shape
+    finally:
+        for g in grids:
+            g.close()
+
+
+def test_batch_graph_replays(gpu):
+    """The same captured graph replayed several times, with eager work between."""
+    grids, stencils, fields = _make(SHAPES[:5], seed=3)
+    try:
+        with api.Batch(grids, stencils) as b:
+            for rep in range(1, 4):
+                b.run(10, use_graph=True)
+                grids[0].run(stencils[0], 1)  # eager sweep of one grid between replays
+                api.synchronize()
+                for j, (g, s, f) in enumerate(zip(grids, stencils, fields)):
+                    n = 10 * rep + (rep if j == 0 else 0)
+                    assert np.array_equal(g.download(), oracle.run_2d(s, f, n)), (rep, j)
+    finally:
+        for g in grids:
+            g.close()
+
+
+def test_batch_rejects_3d_and_generic(gpu):
+    g3 = api.Grid((8, 8, 8), "f32")
+    g2 = api.Grid((32, 32), "f32")
+    try:
+        with pytest.raises(PSkelError):
+            api.Batch([g3], [K.jacobi3d()])
+        odd = K.Stencil(K.SCALED_SUM, 2, [(0, 2), (0, -2)], coeff=0.5)
+        with pytest.raises(PSkelError):
+            api.Batch([g2], [odd])
+    finally:
+        g3.close()
+        g2.close()
+
+
+def test_batch_256_uneven_grids(gpu):
+    """The full workload shape: 256 grids, sides log-uniform in [64, 2048],
+    half Jacobi f32 half life u8, 200 sweeps in one graph; all bit-exact."""
+    shapes = inputs.batch_shapes(256, seed=5)
+    grids, stencils, fields = _make(shapes, seed=100)
+    try:
+        with api.Batch(grids, stencils) as b:
+            st = b.run(200, use_graph=True)
+            assert st["launches"] == 200
+        for i in range(0, 256, 17):  # spot-check a spread of grids (the oracle is the slow part)
+            assert np.array_equal(grids[i].download(), oracle.run_2d(stencils[i], fields[i], 200)), shapes[i]
+    finally:
+        for g in grids:
+            g.close()

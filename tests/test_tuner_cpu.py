@@ -1,0 +1,64 @@
+"""Tuner host logic that needs no device: candidate spaces, features, heuristic."""
+import numpy as np
+import pytest
+
+from pskel_amd import _lib as L
+from pskel_amd import tuner as T
+
+
+def _q(shape, storage, name, it=100):
+    return T.Query(tuple(shape), storage, name, it)
+
+
+@pytest.mark.parametrize("q,expect_tuned", [
+    (_q((2048, 2048), "f32", "jacobi2d"), True),
+    (_q((8194, 8194), "u8", "life"), True),
+    (_q((4096, 4096), "f32", "diffusion9"), True),
+    (_q((512, 512, 512), "f32", "jacobi3d"), True),
+    (_q((100, 200, 300), "bf16", "jacobi3d"), True),
+])
+def test_candidates_cover_tuned_and_plain(q, expect_tuned):
+    c = T.candidates(q)
+    assert len(c) >= 10
+    assert any(x.variant == L.VARIANT_PLAIN for x in c)
+    assert any(x.variant == L.VARIANT_TUNED for x in c) == expect_tuned
+    keys = [T.cfg_key(x) for x in c]
+    assert len(set(keys)) == len(keys)
+
+
+def test_3d_candidates_fit_lds():
+    for st in ("f32", "bf16"):
+        for c in T.candidates(_q((64, 64, 64), st, "jacobi3d")):
+            if c.variant == L.VARIANT_TUNED:
+                lds = 2 * (4 * c.tile_y + 2) * (64 * c.tile_x + 2 * c.tile_x) * 4
+                assert lds <= 64 * 1024
+                assert (c.tile_x in (1, 2, 4)) if st == "f32" else (c.tile_x in (2, 4, 8))
+
+
+def test_features_shape_and_determinism():
+    q = _q((300, 700), "f32", "jacobi2d", 50)
+    c = T.candidates(q)
+    f1 = T.features(q, c)
+    f2 = T.features(q, c)
+    assert f1.shape == (len(c), L.TUNER_NFEAT) and np.array_equal(f1, f2)
+    assert np.all(np.isfinite(f1))
+    # candidates differ in their features
+    assert len({tuple(r) for r in f1}) == len(c)
+    # the query part encodes log2 dims and iterations
+    assert abs(f1[0, 0] - np.log2(700)) < 1e-5 and abs(f1[0, 1] - np.log2(300)) < 1e-5
+    assert abs(f1[0, 8] - np.log2(50)) < 1e-5
+
+
+def test_heuristic_picks_a_candidate():
+    for q in (_q((2048, 2048), "f32", "jacobi2d"), _q((512, 512, 512), "f32", "jacobi3d"),
+              _q((64, 64), "u8", "life", 3)):
+        h = T.heuristic_choice(q)
+        assert T.cfg_key(h) in {T.cfg_key(c) for c in T.candidates(q)}
+
+
+def test_random_queries_reproducible():
+    a = T.random_queries(12, 7)
+    b = T.random_queries(12, 7)
+    assert a == b
+    for q in a:
+        assert all(s >= 32 for s in q.shape) and 10 <= q.iterations <= 1000
