@@ -33,8 +33,10 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     # name: (shape, storage, stencil, sweeps per step, generator)
     "jacobi3d": dict(shape=(512, 512, 512), storage="f32", sweeps=1000, label="jacobi3d-7pt 512^3 f32 x1000"),
-    "jacobi3d_bf16": dict(shape=(512, 512, 512), storage="bf16", sweeps=1000, label="jacobi3d-7pt 512^3 bf16-storage x1000"),
-    "jacobi3d_f16": dict(shape=(512, 512, 512), storage="f16", sweeps=1000, label="jacobi3d-7pt 512^3 f16-storage x1000"),
+    "jacobi3d_bf16": dict(shape=(512, 512, 512), storage="bf16", sweeps=1000,
+                          label="jacobi3d-7pt 512^3 bf16-storage (stochastic rounding) x1000"),
+    "jacobi3d_f16": dict(shape=(512, 512, 512), storage="f16", sweeps=1000,
+                         label="jacobi3d-7pt 512^3 f16-storage (stochastic rounding) x1000"),
     "life": dict(shape=(8194, 8194), storage="u8", sweeps=500, label="life B3/S23 8192^2 u8 x500"),
     "jacobi2d": dict(shape=(2048, 2048), storage="f32", sweeps=100, label="jacobi2d-5pt 2048^2 f32 x100"),
     "diffusion9": dict(shape=(4096, 4096), storage="f32", sweeps=1000, label="diffusion9 4096^2 f32 x1000"),
@@ -43,8 +45,8 @@ WORKLOADS = {
 # Default tuned launch shapes (overridden by --tile / --time-tile).
 TUNED = {
     "jacobi3d": dict(tile=(4, 1, 128)),
-    "jacobi3d_bf16": dict(tile=(4, 1, 32)),
-    "jacobi3d_f16": dict(tile=(4, 1, 64)),
+    "jacobi3d_bf16": dict(tile=(2, 2, 32)),
+    "jacobi3d_f16": dict(tile=(4, 2, 32)),
     "life": dict(tile=(0, 16, 0), time_tile=4),
     "jacobi2d": dict(tile=(0, 16, 0), time_tile=4),
     "diffusion9": dict(tile=(0, 16, 0), time_tile=3),
@@ -72,9 +74,16 @@ def field_of(name):
 
 
 def store_offset_of(name):
-    # 16-bit storage holds u - 0.5: the interior relaxes towards the mean 0.5,
-    # so the stored deviations are small and keep more relative precision
-    return 0.5 if name in ("jacobi3d_bf16", "jacobi3d_f16") else 0.0
+    # 16-bit storage holds u - 0.75: the solution relaxes from the interior mean
+    # 0.5 towards the face value 1.0, so deviations from their midpoint stay
+    # small and keep more absolute precision
+    return 0.75 if name in ("jacobi3d_bf16", "jacobi3d_f16") else 0.0
+
+
+def rounding_of(name):
+    # 16-bit stores round stochastically: nearest rounding stalls the slow
+    # relaxation once updates drop below half an ulp
+    return "stochastic" if name in ("jacobi3d_bf16", "jacobi3d_f16") else "nearest"
 
 
 def measure(name, variant, steps, warmup, field=None, run_kw=None):
@@ -88,7 +97,8 @@ def measure(name, variant, steps, warmup, field=None, run_kw=None):
         kw = dict(TUNED[name])
     f = field_of(name) if field is None else field
     s = stencil_of(name)
-    with api.Grid(w["shape"], w["storage"], store_offset=store_offset_of(name)) as g:
+    with api.Grid(w["shape"], w["storage"], store_offset=store_offset_of(name), rounding=rounding_of(name),
+                  seed=17) as g:
         g.upload_pair(f)
         for _ in range(warmup):
             g.run(s, w["sweeps"], variant=variant, **kw)
@@ -207,6 +217,50 @@ def measure_batch(steps, warmup, sweeps=200):
     }
 
 
+def accuracy_3d(sweeps=1000, every=50, n=512):
+    """Jacobi-3D n^3 x sweeps against the FP64 CPU oracle: f32 storage, and bf16
+    and f16 storage of u - 0.75 with stochastic rounding (f32 arithmetic, tuned
+    kernels), plus three bf16 controls (nearest rounding; offset 0.5; no
+    offset).  Reports the max absolute
+    and relative-L2 error of each GPU result and the FP64 residuals the device
+    reduces every `every` sweeps next to the oracle's."""
+    import oracle
+    from pskel_amd import api
+    from pskel_amd import inputs
+    from pskel_amd import kernels as K
+
+    f64 = inputs.jacobi3d_field(n, seed=4, dtype=np.float64)
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or oracle.num_procs(), 16)
+    t0 = time.perf_counter()
+    want, want_res = oracle.run_3d(K.jacobi3d(), f64, sweeps, threads=threads, residual_every=every)
+    oracle_s = time.perf_counter() - t0
+    f32 = f64.astype(np.float32)
+    norm = float(np.linalg.norm(want))
+    out = {"oracle": f"FP64 runIterativeCPU restatement, {threads} threads, {oracle_s:.0f} s",
+           "grid": f"{n}^3 x {sweeps}", "residual_every": every}
+    for name, storage, off, rnd in (("f32", "f32", 0.0, "nearest"), ("bf16", "bf16", 0.75, "stochastic"),
+                                    ("f16", "f16", 0.75, "stochastic"), ("bf16_nearest", "bf16", 0.75, "nearest"),
+                                    ("bf16_offset_0.5", "bf16", 0.5, "stochastic"),
+                                    ("bf16_no_offset", "bf16", 0.0, "stochastic")):
+        kw = dict(TUNED["jacobi3d" if storage == "f32" else "jacobi3d_" + storage])
+        with api.Grid(f32.shape, storage, store_offset=off, rounding=rnd, seed=17) as g:
+            g.upload_pair(f32)
+            st = g.run(K.jacobi3d(), sweeps, variant="tuned", residual_every=every, **kw)
+            got = g.download(np.float64)
+        d = got - want
+        res = np.array(st["residuals"])
+        out[name] = {
+            "storage_offset": off,
+            "rounding": rnd,
+            "max_abs_err": float(np.abs(d).max()),
+            "rel_l2_err": float(np.linalg.norm(d) / norm),
+            "residual_first_last": [float(res[0]), float(res[-1])] if len(res) else None,
+            "residual_rel_diff_vs_fp64_max": float(np.max(np.abs(res - want_res) / want_res)) if len(res) else None,
+        }
+    out["fp64_residual_first_last"] = [float(want_res[0]), float(want_res[-1])]
+    return out
+
+
 def cpu_baseline(name, seconds_target=12.0):
     """The oracle (runIterativeCPU restatement, OpenMP) on a bounded sample:
     the full grid for as many sweeps as fit about `seconds_target`."""
@@ -285,14 +339,15 @@ def main(argv=None):
     ap.add_argument("--time-tile", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64-oracle error report")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the plain copy kernel on two buffers of this size (PMC calibration)")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the grid would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (a grid would shard "
+              "into h-slabs with one-plane halo exchanges); run with --gpus 1", file=sys.stderr)
         return 2
 
     from pskel_amd import _lib, api
@@ -350,6 +405,11 @@ def main(argv=None):
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.workload)
+    acc = None
+    if not args.no_accuracy and args.workload.startswith("jacobi3d") and not args.no_extra:
+        acc = accuracy_3d()
+        print(f"# accuracy vs FP64: " + ", ".join(f"{k} rel-L2 {v['rel_l2_err']:.2e}" for k, v in acc.items()
+                                                  if isinstance(v, dict)), file=sys.stderr, flush=True)
 
     info = api.device_info(0)
     line = {
@@ -371,6 +431,11 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": cpu,
         "workloads": extra,
+        "accuracy_vs_fp64": acc,
+        "bf16_storage_speedup_vs_f32": (round(extra["jacobi3d_bf16:tuned"]["cell_updates_per_s"] /
+                                              head["cell_updates_per_s"], 3)
+                                        if (args.workload, args.variant) == ("jacobi3d", "tuned")
+                                        and "jacobi3d_bf16:tuned" in extra else None),
         "device": {"arch": info["name"], "cus": info["cus"]},
     }
     print(json.dumps(line))

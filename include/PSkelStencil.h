@@ -90,8 +90,13 @@ struct GpuState {
     int device = 0;
     int storage = 0;  // 0: the host element type
     float storeOffset = 0.0f;
+    int rounding = PSKEL_ROUND_NEAREST;
+    uint32_t seed = 0;
     size_t dims[3] = {0, 0, 0};
     int gridStorage = 0;
+    int gridRounding = 0;
+    uint32_t gridSeed = 0;
+    float gridOffset = 0.0f;
     size_t residualEvery = 0;
     std::vector<double> residuals;
     bool haveCfg = false;
@@ -272,11 +277,15 @@ public:
     // ---- MI355X extensions ------------------------------------------------
     void setDevice(int device) { gpu->device = device; }
     // Device storage of the grid: PSKEL_DTYPE_BF16 / F16 keep (u - offset) in
-    // 16 bits with f32 arithmetic (3D); 0 = the host element type.
-    void setDeviceStorage(int storage, float offset = 0.0f)
+    // 16 bits with f32 arithmetic (3D); 0 = the host element type.  `rounding`
+    // is PSKEL_ROUND_NEAREST or PSKEL_ROUND_STOCHASTIC (reproducible from `seed`).
+    void setDeviceStorage(int storage, float offset = 0.0f, int rounding = PSKEL_ROUND_NEAREST,
+                          uint32_t seed = 0)
     {
         gpu->storage = storage;
         gpu->storeOffset = offset;
+        gpu->rounding = rounding;
+        gpu->seed = seed;
     }
     // FP64 residual ||u^k - u^(k-1)||_2 every n sweeps of the next GPU runs.
     void setResidualInterval(size_t n) { gpu->residualEvery = n; }
@@ -299,7 +308,8 @@ protected:
         if (storage == PSKEL_DTYPE_F64 || storage == 0)
             gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);  // no FP64 sweeps on the device
         const size_t d0 = input.getWidth(), d1 = input.getHeight(), d2 = input.getDepth();
-        if (s.grid && (s.dims[0] != d0 || s.dims[1] != d1 || s.dims[2] != d2 || s.gridStorage != storage)) {
+        if (s.grid && (s.dims[0] != d0 || s.dims[1] != d1 || s.dims[2] != d2 || s.gridStorage != storage ||
+                       s.gridRounding != s.rounding || s.gridSeed != s.seed || s.gridOffset != s.storeOffset)) {
             pskel_grid_destroy(s.grid);
             s.grid = nullptr;
         }
@@ -312,11 +322,16 @@ protected:
             gd.depth = dims() == 3 ? d2 : 1;
             gd.store_offset = s.storeOffset;
             gd.device = s.device;
+            gd.rounding = s.rounding;
+            gd.seed = s.seed;
             gpuErrchk(pskel_grid_create(&gd, &s.grid));
             s.dims[0] = d0;
             s.dims[1] = d1;
             s.dims[2] = d2;
             s.gridStorage = storage;
+            s.gridRounding = s.rounding;
+            s.gridSeed = s.seed;
+            s.gridOffset = s.storeOffset;
         }
     }
 

@@ -116,6 +116,16 @@ typedef struct pskel_run_stats {
 
 typedef struct pskel_grid pskel_grid_t;
 
+/* Rounding of BF16/F16 stores.  STOCHASTIC rounds the magnitude up with
+ * probability equal to the dropped fraction of an ulp, from random bits that
+ * are a hash of (seed, index of the sweep since the last INPUT upload, unpitched
+ * cell index (h*W + w)*D + d): unbiased, so slow relaxation does not stall below
+ * half an ulp, and reproducible run to run. */
+typedef enum pskel_rounding {
+    PSKEL_ROUND_NEAREST = 0,
+    PSKEL_ROUND_STOCHASTIC = 1
+} pskel_rounding_t;
+
 typedef struct pskel_grid_desc {
     int32_t ndim;        /* 2 or 3 */
     int32_t storage;     /* pskel_dtype_t on the device: U8, F32, BF16 or F16 */
@@ -124,6 +134,8 @@ typedef struct pskel_grid_desc {
     size_t depth;        /* depth (1 for 2D); the contiguous axis in 3D (include/PSkelArray.hpp:642) */
     float store_offset;  /* BF16/F16 storage holds (u - store_offset) */
     int32_t device;      /* HIP device ordinal */
+    int32_t rounding;    /* pskel_rounding_t of BF16/F16 stores */
+    uint32_t seed;       /* stochastic rounding seed */
 } pskel_grid_desc_t;
 
 typedef enum pskel_buffer {
@@ -188,7 +200,8 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* grid, const pskel_stencil_desc_
 /* Record one sweep launched outside the library on the grid's buffers (the
  * drop-in headers' device path for user stencil kernels with no descriptor):
  * the previous buffer becomes the current state, as runIterativeGPU's
- * in/out swap does (include/PSkelStencil.hpp:1078-1082). */
+ * in/out swap does (include/PSkelStencil.hpp:1078-1082), and counts it in the
+ * sweep index that stochastic rounding draws from. */
 pskel_status_t pskel_grid_swap(pskel_grid_t* grid);
 
 /* FP64 ||current - previous||_2 over every cell (the convergence test the

@@ -20,7 +20,9 @@ Pinning status per function (see tests/test_oracle_*.py):
   run_3d f32/f64   separable eigenmode decay, u == 1 fixed point, halo
                    boundary closed form, residual closed form
   run_3d r16       conversion round trips against numpy's float16, RNE ties,
-                   equality with the f32 runner on exactly representable data
+                   equality with the f32 runner on exactly representable data;
+                   stochastic rounding: exhaustive probability counts, closed
+                   forms, recorded gfx950 v_cvt_sr outputs, unbiasedness
   mlp              finite differences of the loss (central, FP64)
 """
 from __future__ import annotations
@@ -78,6 +80,14 @@ def lib():
             f.restype = SZ
         l.oracle_run_iterative_3d_r16.argtypes = [D, I, P, P, SZ, SZ, SZ, SZ, I, SZ, P]
         l.oracle_run_iterative_3d_r16.restype = SZ
+        l.oracle_run_iterative_3d_r16_mode.argtypes = [D, I, I, C.c_uint32, C.c_uint64, P, P, SZ, SZ, SZ, SZ, I, SZ, P]
+        l.oracle_run_iterative_3d_r16_mode.restype = SZ
+        l.oracle_sr_bits.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64]
+        l.oracle_sr_bits.restype = C.c_uint32
+        l.oracle_f32_to_bf16_sr.argtypes = [C.c_float, C.c_uint32]
+        l.oracle_f32_to_bf16_sr.restype = C.c_uint16
+        l.oracle_f32_to_f16_sr.argtypes = [C.c_float, C.c_uint32]
+        l.oracle_f32_to_f16_sr.restype = C.c_uint16
         l.oracle_range.argtypes = [D, I]
         l.oracle_range.restype = C.c_int32
         l.oracle_f32_to_bf16.argtypes = [C.c_float]
@@ -156,8 +166,12 @@ def run_3d(stencil, inp: np.ndarray, iterations: int, out: np.ndarray | None = N
     return b, res[:n]
 
 
+ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
+
+
 def run_3d_r16(stencil, store: int, inp_words: np.ndarray, iterations: int, store_offset: float = 0.0,
-               out_words: np.ndarray | None = None, threads: int = 0, residual_every: int = 0):
+               out_words: np.ndarray | None = None, threads: int = 0, residual_every: int = 0,
+               rounding: int = ROUND_NEAREST, seed: int = 0, first_sweep: int = 0):
     """(result words, residuals) with bf16/f16 storage of (u - store_offset)."""
     l = lib()
     d = desc_of(stencil, store_offset)
@@ -166,9 +180,21 @@ def run_3d_r16(stencil, store: int, inp_words: np.ndarray, iterations: int, stor
     H, W, D = a.shape
     nres = iterations // residual_every if residual_every else 0
     res = np.zeros(max(nres, 1), np.float64)
-    n = l.oracle_run_iterative_3d_r16(C.byref(d), int(store), _ptr(a), _ptr(b), W, H, D, int(iterations),
-                                      int(threads), int(residual_every), _ptr(res))
+    n = l.oracle_run_iterative_3d_r16_mode(C.byref(d), int(store), int(rounding), int(seed), int(first_sweep),
+                                           _ptr(a), _ptr(b), W, H, D, int(iterations), int(threads),
+                                           int(residual_every), _ptr(res))
     return b, res[:n]
+
+
+def sr_bits(seed: int, sweep: int, h: int, wd: int) -> int:
+    """32-bit random word of cell (h, w, d) with wd = w*D + d."""
+    return lib().oracle_sr_bits(seed, sweep, h, wd)
+
+
+def encode16_sr(x: float, store: int, r: int) -> int:
+    l = lib()
+    f = l.oracle_f32_to_bf16_sr if store == STORE_BF16 else l.oracle_f32_to_f16_sr
+    return f(float(np.float32(x)), int(r))
 
 
 def encode16(x: np.ndarray, store: int, offset: float = 0.0) -> np.ndarray:

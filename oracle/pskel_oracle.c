@@ -92,6 +92,81 @@ float oracle_f16_to_f32(uint16_t h)
     return u2f(sign | ((e - 15u + 127u) << 23) | (m << 13));
 }
 
+/* Stochastic rounding.  32-bit integer finaliser (a murmur3-style avalanche;
+ * a bijection with mix32(0) = 0). */
+static uint32_t mix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
+uint32_t oracle_sr_key(uint32_t seed, uint64_t sweep)
+{
+    return mix32(seed + 0x9e3779b9u * ((uint32_t)sweep + 1u));
+}
+
+/* Random word of cell (h, w, d) in the sweep with key `key`: A(w*D + d) ^ B(h),
+ * two independent hash streams of the in-plane index and of the plane.  For
+ * two distinct cells the words are pairwise independent (distinct in-plane
+ * index: independent A terms; same one: the B terms differ), so the rounding
+ * errors are uncorrelated as with a fresh draw per cell. */
+uint32_t oracle_sr_plane_bits(uint32_t key, uint64_t h)
+{
+    return mix32(mix32(key + 0x632be5abu) ^ (uint32_t)h);
+}
+
+uint32_t oracle_sr_inplane_bits(uint32_t key, uint64_t wd)
+{
+    return mix32(key ^ (uint32_t)wd);
+}
+
+uint32_t oracle_sr_bits(uint32_t seed, uint64_t sweep, uint64_t h, uint64_t wd)
+{
+    const uint32_t key = oracle_sr_key(seed, sweep);
+    return oracle_sr_inplane_bits(key, wd) ^ oracle_sr_plane_bits(key, h);
+}
+
+/* bf16: add the top 16 bits of r below the kept bits and truncate, so the
+ * magnitude rounds up with probability = dropped fraction of an ulp (to
+ * 2^-16); the carry may reach the exponent, and overflow gives inf.  NaN keeps
+ * its top payload bits with the quiet bit set. */
+uint16_t oracle_f32_to_bf16_sr(float x, uint32_t r)
+{
+    const uint32_t u = f2u(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + (r >> 16)) >> 16);
+}
+
+/* f16: the top 13 bits of r are added to the 13 dropped mantissa bits of the
+ * f32 magnitude, which is then truncated (exact in f16; 65536 and above give
+ * inf).  Below 2^-14 the f16 spacing is 2^-24: with s = |x| 2^24 the result is
+ * floor(s) + 1 exactly when floor(frac(s) 2^13) + (r >> 19) >= 2^13, so the
+ * dropped fraction counts to 13 bits as in the normal range.  NaN keeps its
+ * top payload bits with the quiet bit set. */
+uint16_t oracle_f32_to_f16_sr(float x, uint32_t r)
+{
+    const uint32_t u = f2u(x);
+    const uint32_t a = u & 0x7fffffffu;
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const uint32_t r13 = r >> 19;
+    if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u | ((a >> 13) & 0x3ffu));
+    if (a == 0x7f800000u) return (uint16_t)(sign | 0x7c00u);
+    if (a >= 0x38800000u) {
+        const uint32_t t = (a + r13) & ~0x1fffu;
+        if (t >= 0x47800000u) return (uint16_t)(sign | 0x7c00u);
+        const uint32_t e = (t >> 23) - 127u + 15u;
+        return (uint16_t)(sign | (e << 10) | ((t >> 13) & 0x3ffu));
+    }
+    const double sv = (double)u2f(a) * 16777216.0;   /* exact */
+    const double fl = floor(sv);
+    const uint32_t k = (uint32_t)floor((sv - fl) * 8192.0);
+    return (uint16_t)(sign | (uint32_t)(fl + (k + r13 >= 8192u ? 1.0 : 0.0)));
+}
+
 /* ------------------------------------------------------------------------ */
 /* 2D: Stencil2D::runOpenMP over the interior [r, N-r) (PSkelStencil.hpp:2235) */
 /* ------------------------------------------------------------------------ */
@@ -323,15 +398,17 @@ typedef uint16_t (*enc_fn)(float);
  * order -> subtract offset -> round to the storage format. */
 static double sweep3d_r16(const plan3d* p, int store, float off, const uint16_t* in,
                           uint16_t* out, size_t W, size_t H, size_t D, int th, int want_res,
-                          float halo)
+                          float halo, int rounding, uint32_t seed, uint64_t sweep)
 {
     const dec_fn dec = store == ORACLE_STORE_BF16 ? oracle_bf16_to_f32 : oracle_f16_to_f32;
     const enc_fn enc = store == ORACLE_STORE_BF16 ? oracle_f32_to_bf16 : oracle_f32_to_f16;
     const size_t r = p->r;
+    const uint32_t key = oracle_sr_key(seed, sweep);
     double* rows = want_res ? (double*)calloc(H, sizeof(double)) : NULL;
 #pragma omp parallel for num_threads(th) schedule(static)
     for (size_t h = 0; h < H; ++h) {
         double rs = 0.0;
+        const uint32_t plane_bits = oracle_sr_plane_bits(key, h);
         for (size_t w = 0; w < W; ++w) {
             const size_t base = (h * W + w) * D;
             for (size_t dd = 0; dd < D; ++dd) {
@@ -352,7 +429,13 @@ static double sweep3d_r16(const plan3d* p, int store, float off, const uint16_t*
                         else acc = (i == 0) ? p->wf[i] * v : acc + p->wf[i] * v;
                     }
                     if (p->kind == ORACLE_KIND_SCALED_SUM) acc = p->cf * acc;
-                    sv = enc(acc - off);
+                    if (rounding == ORACLE_ROUND_STOCHASTIC) {
+                        const uint32_t rb = oracle_sr_inplane_bits(key, w * D + dd) ^ plane_bits;
+                        sv = store == ORACLE_STORE_BF16 ? oracle_f32_to_bf16_sr(acc - off, rb)
+                                                        : oracle_f32_to_f16_sr(acc - off, rb);
+                    } else {
+                        sv = enc(acc - off);
+                    }
                 }
                 if (want_res) {
                     double dv = (double)dec(sv) - (double)dec(in[base + dd]);
@@ -415,10 +498,10 @@ size_t oracle_run_iterative_3d_f64(const oracle_desc* d, double* input, double* 
     return nres;
 }
 
-size_t oracle_run_iterative_3d_r16(const oracle_desc* d, int store, uint16_t* input,
-                                   uint16_t* output, size_t width, size_t height, size_t depth,
-                                   size_t iterations, int threads, size_t residual_every,
-                                   double* residuals)
+size_t oracle_run_iterative_3d_r16_mode(const oracle_desc* d, int store, int rounding, uint32_t seed,
+                                        uint64_t first_sweep, uint16_t* input, uint16_t* output,
+                                        size_t width, size_t height, size_t depth, size_t iterations,
+                                        int threads, size_t residual_every, double* residuals)
 {
     plan3d p;
     plan3d_init(&p, d, width, depth);
@@ -428,16 +511,28 @@ size_t oracle_run_iterative_3d_r16(const oracle_desc* d, int store, uint16_t* in
     for (size_t it = 0; it < iterations; ++it) {
         int want = residual_every && ((it + 1) % residual_every == 0);
         double rn;
+        const uint64_t sw = first_sweep + it;
         if (it % 2 == 0)
-            rn = sweep3d_r16(&p, store, d->store_offset, input, output, width, height, depth, th, want, d->halo_value);
+            rn = sweep3d_r16(&p, store, d->store_offset, input, output, width, height, depth, th, want,
+                             d->halo_value, rounding, seed, sw);
         else
-            rn = sweep3d_r16(&p, store, d->store_offset, output, input, width, height, depth, th, want, d->halo_value);
+            rn = sweep3d_r16(&p, store, d->store_offset, output, input, width, height, depth, th, want,
+                             d->halo_value, rounding, seed, sw);
         if (want && residuals) residuals[nres] = rn;
         if (want) nres++;
     }
     if (iterations % 2 == 0) memcpy(output, input, width * height * depth * sizeof(uint16_t));
     g_last_seconds = omp_get_wtime() - t0;
     return nres;
+}
+
+size_t oracle_run_iterative_3d_r16(const oracle_desc* d, int store, uint16_t* input,
+                                   uint16_t* output, size_t width, size_t height, size_t depth,
+                                   size_t iterations, int threads, size_t residual_every,
+                                   double* residuals)
+{
+    return oracle_run_iterative_3d_r16_mode(d, store, ORACLE_ROUND_NEAREST, 0, 0, input, output, width, height,
+                                            depth, iterations, threads, residual_every, residuals);
 }
 
 /* ------------------------------------------------------------------------ */

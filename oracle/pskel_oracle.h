@@ -47,6 +47,23 @@ enum {
 
 enum { ORACLE_STORE_F32 = 0, ORACLE_STORE_BF16 = 1, ORACLE_STORE_F16 = 2 };
 
+/* Rounding of 16-bit stores: to nearest even, or stochastic.  Stochastic
+ * rounding rounds the magnitude up with probability equal to the fraction of
+ * an ulp being dropped (counted to 2^-16 for bf16, 2^-13 for f16), from the top
+ * bits of a 32-bit random word r = oracle_sr_bits(seed, sweep, h, w*D + d):
+ * sweep is the 0-based index of the sweep producing the value (counted from
+ * the upload).  No reference counterpart (the reference has no reduced-precision
+ * path); the conversions are pinned by exhaustive probability counts, closed
+ * forms and recorded outputs of gfx950's v_cvt_sr_bf16_f32 / v_cvt_sr_f16_f32,
+ * whose rounding they restate. */
+enum { ORACLE_ROUND_NEAREST = 0, ORACLE_ROUND_STOCHASTIC = 1 };
+uint32_t oracle_sr_key(uint32_t seed, uint64_t sweep);
+uint32_t oracle_sr_plane_bits(uint32_t key, uint64_t h);
+uint32_t oracle_sr_inplane_bits(uint32_t key, uint64_t wd);
+uint32_t oracle_sr_bits(uint32_t seed, uint64_t sweep, uint64_t h, uint64_t wd);
+uint16_t oracle_f32_to_bf16_sr(float x, uint32_t r);
+uint16_t oracle_f32_to_f16_sr(float x, uint32_t r);
+
 typedef struct oracle_desc {
     int32_t kind;
     int32_t npoints;
@@ -90,6 +107,11 @@ size_t oracle_run_iterative_3d_r16(const oracle_desc* d, int store, uint16_t* in
                                    uint16_t* output, size_t width, size_t height, size_t depth,
                                    size_t iterations, int threads, size_t residual_every,
                                    double* residuals);
+/* Same with a rounding mode (ORACLE_ROUND_*), seed and the index of the first sweep. */
+size_t oracle_run_iterative_3d_r16_mode(const oracle_desc* d, int store, int rounding, uint32_t seed,
+                                        uint64_t first_sweep, uint16_t* input, uint16_t* output,
+                                        size_t width, size_t height, size_t depth, size_t iterations,
+                                        int threads, size_t residual_every, double* residuals);
 
 /* Storage conversions, round-to-nearest-even, NaN preserved. */
 uint16_t oracle_f32_to_bf16(float x);

@@ -39,6 +39,9 @@ BOUNDARY_HALO = 1
 VARIANT_PLAIN = 0
 VARIANT_TUNED = 1
 
+ROUND_NEAREST = 0
+ROUND_STOCHASTIC = 1
+
 BUFFER_INPUT = 0
 BUFFER_OUTPUT = 1
 
@@ -93,6 +96,8 @@ class GridDesc(C.Structure):
         ("depth", C.c_size_t),
         ("store_offset", C.c_float),
         ("device", C.c_int32),
+        ("rounding", C.c_int32),
+        ("seed", C.c_uint32),
     ]
 
 

@@ -49,7 +49,7 @@ class Grid:
     """A device-resident grid pair (input/output ping-pong) on one GPU."""
 
     def __init__(self, shape: Sequence[int], storage: str = "f32", store_offset: float = 0.0,
-                 device: int = 0):
+                 device: int = 0, rounding: str = "nearest", seed: int = 0):
         if len(shape) not in (2, 3):
             raise ValueError("shape is (height, width) or (height, width, depth)")
         self.shape = tuple(int(s) for s in shape)
@@ -64,6 +64,8 @@ class Grid:
         gd.depth = self.shape[2] if self.ndim == 3 else 1
         gd.store_offset = store_offset
         gd.device = device
+        gd.rounding = {"nearest": L.ROUND_NEAREST, "stochastic": L.ROUND_STOCHASTIC}[rounding]
+        gd.seed = int(seed) & 0xFFFFFFFF
         h = C.c_void_p()
         L.check(L.lib().pskel_grid_create(C.byref(gd), C.byref(h)))
         self._h = h

@@ -246,8 +246,10 @@ pskel_status_t pskel_batch_run(pskel_batch_t* b, size_t iterations, int32_t use_
     PSKEL_HIP(hipEventSynchronize(b->ev1));
     float ms = 0.f;
     PSKEL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-    for (pskel_grid_t* g : b->grids)
+    for (pskel_grid_t* g : b->grids) {
         for (size_t it = 0; it < iterations % 2; ++it) advance_state(*g, 1);
+        g->sweeps_done += iterations;
+    }
     if (stats) {
         stats->elapsed_ms = ms;
         stats->launches = (int64_t)iterations;

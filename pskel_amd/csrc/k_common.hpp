@@ -13,9 +13,48 @@ namespace pskel {
 constexpr int kWave = 64;
 
 // ---------------------------------------------------------------------------
-// Storage formats.  Arithmetic is always f32; 16-bit formats store (u - off)
-// and round to nearest even on the way out.
+// Stochastic rounding words.  mix32 is a murmur3-style avalanche.  A sweep's
+// key mixes the seed with the sweep index; the 32-bit random word of cell
+// (h, w, d) is A(w*D + d) ^ B(h), two independent hash streams of the
+// in-plane index and of the plane: pairwise independent between distinct
+// cells, and a kernel marching along h hashes A once per cell of its tile and
+// B once per plane (a scalar).
 // ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
+__host__ __device__ __forceinline__ uint32_t sweep_key(uint32_t seed, uint64_t sweep)
+{
+    return mix32(seed + 0x9e3779b9u * ((uint32_t)sweep + 1u));
+}
+
+__host__ __device__ __forceinline__ uint32_t sr_plane_bits(uint32_t key, uint32_t h)
+{
+    return mix32(mix32(key + 0x632be5abu) ^ h);
+}
+
+__host__ __device__ __forceinline__ uint32_t sr_inplane_bits(uint32_t key, uint32_t wd)
+{
+    return mix32(key ^ wd);
+}
+
+// ---------------------------------------------------------------------------
+// Storage formats.  Arithmetic is always f32; 16-bit formats store (u - off),
+// rounded to nearest even (enc) or stochastically (enc_sr) on the way out.
+// Stochastic rounding is gfx950's v_cvt_sr_{bf16,f16}_f32: the top 16 (bf16)
+// or 13 (f16) bits of the random word are added below the kept bits of the
+// magnitude, which is then truncated, so it rounds up with probability equal
+// to the dropped fraction of an ulp.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
 template <int ST>
 struct Stor;
 
@@ -25,6 +64,7 @@ struct Stor<PSKEL_DTYPE_F32> {
     __device__ __forceinline__ static float dec(T v, float) { return v; }
     __device__ __forceinline__ static float dec_raw(T v) { return v; }
     __device__ __forceinline__ static T enc(float v, float) { return v; }
+    __device__ __forceinline__ static T enc_sr(float v, float, uint32_t) { return v; }
 };
 
 template <>
@@ -38,6 +78,12 @@ struct Stor<PSKEL_DTYPE_BF16> {
         const __bf16 b = (__bf16)(v - off);
         return __builtin_bit_cast(T, b);
     }
+    __device__ __forceinline__ static T enc_sr(float v, float off, uint32_t r)
+    {
+        bf16x2_t o = {};
+        o = __builtin_amdgcn_cvt_sr_bf16_f32(o, v - off, r, false);
+        return __builtin_bit_cast(T, o[0]);
+    }
 };
 
 template <>
@@ -48,6 +94,12 @@ struct Stor<PSKEL_DTYPE_F16> {
     __device__ __forceinline__ static T enc(float v, float off)
     {
         return __half_as_ushort(__float2half_rn(v - off));
+    }
+    __device__ __forceinline__ static T enc_sr(float v, float off, uint32_t r)
+    {
+        f16x2_t o = {};
+        o = __builtin_amdgcn_cvt_sr_f16_f32(o, v - off, r, false);
+        return __builtin_bit_cast(T, o[0]);
     }
 };
 

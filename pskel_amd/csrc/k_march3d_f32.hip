@@ -1,0 +1,12 @@
+// Marching 3D kernels with f32 storage (the template is in k_march3d.hpp).
+#define PSKEL_MARCH_INSTANTIATE
+#include "k_march3d.hpp"
+
+namespace pskel {
+
+hipError_t march3d_f32(const Sweep3D& s, const Points& p, const MarchShape& m, hipStream_t st)
+{
+    return march_vd<PSKEL_DTYPE_F32>(s, p, m, st);
+}
+
+}  // namespace pskel

@@ -213,6 +213,9 @@ Sweep2D make_sweep2d(const pskel_grid_t* g, const void* src, void* dst, const vo
 Sweep3D make_sweep3d(const pskel_grid_t* g, const void* src, void* dst, int radius, double* partials)
 {
     Sweep3D s;
+    s.rounding = g->rounding;
+    s.seed = g->seed;
+    s.sweep = g->sweeps_done;
     s.src = src;
     s.dst = dst;
     s.W = (int)g->W;
@@ -406,6 +409,7 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
         if (e != hipSuccess) return hip_fail(e, "stencil launch");
         nl++;
         advance_state(*g, k);
+        g->sweeps_done += k;
         if (want_res) {
             if (!fused_res) {
                 const size_t rowlen = g->ndim == 2 ? g->W : g->D;
@@ -523,6 +527,17 @@ pskel_status_t pskel_grid_create(const pskel_grid_desc_t* desc, pskel_grid_t** o
     g->D = D;
     g->device = desc->device;
     g->store_offset = desc->store_offset;
+    if (desc->rounding != PSKEL_ROUND_NEAREST && desc->rounding != PSKEL_ROUND_STOCHASTIC) {
+        delete g;
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown rounding mode");
+    }
+    // the random word of a cell hashes its 32-bit in-plane index w*D + d
+    if (desc->rounding == PSKEL_ROUND_STOCHASTIC && (uint64_t)desc->width * desc->depth >= (1ull << 32)) {
+        delete g;
+        return fail(PSKEL_ERR_UNSUPPORTED, "stochastic rounding needs width * depth < 2^32");
+    }
+    g->rounding = desc->rounding;
+    g->seed = desc->seed;
     // rows are padded to whole 1 KiB segments: the widest tuned tile, so vector
     // loads of a tile never leave the allocation and every row is aligned
     const size_t row = desc->ndim == 2 ? W : D;
@@ -577,6 +592,7 @@ pskel_status_t pskel_grid_upload(pskel_grid_t* g, int32_t which, const void* hos
     PSKEL_HIP(hipSetDevice(g->device));
     // uploading resets the ping-pong: INPUT is the current state, OUTPUT the other
     g->parity = 0;
+    if (which == PSKEL_BUFFER_INPUT) g->sweeps_done = 0;
     g->spare_parity = -1;
     void* dbuf = which == PSKEL_BUFFER_INPUT ? g->buf[0] : g->buf[1];
     const size_t n = cells(g);
@@ -739,6 +755,8 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
         kc.stream = nullptr;
         const int sp = g->spare_parity * 2 + g->parity;
         const void* ptrs[5] = {g->buf[0], g->buf[1], g->buf[2], g->d_res, g->d_partials};
+        // stochastic rounding bakes the sweep indices into the launches
+        const uint64_t sweep0 = g->rounding == PSKEL_ROUND_STOCHASTIC ? g->sweeps_done : 0;
         std::vector<unsigned char> key;
         auto put = [&key](const void* p, size_t n) {
             key.insert(key.end(), (const unsigned char*)p, (const unsigned char*)p + n);
@@ -748,6 +766,7 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
         put(&iterations, sizeof(iterations));
         put(ptrs, sizeof(ptrs));
         put(&sp, sizeof(sp));
+        put(&sweep0, sizeof(sweep0));
         GraphEntry* hit = nullptr;
         for (auto& ge : g->graphs)
             if (ge.key == key) hit = &ge;
@@ -797,6 +816,7 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
             const size_t k = next_k(done, iterations, kmax, every);
             if (k % 2 == 0 && g->ndim == 2) g->spare_parity = g->parity;  // ring copy in the graph
             advance_state(*g, k);
+            g->sweeps_done += k;
             done += k;
         }
     }
@@ -821,6 +841,7 @@ pskel_status_t pskel_grid_swap(pskel_grid_t* g)
     if (st != PSKEL_OK) return st;
     std::swap(g->buf[0], g->buf[1]);
     g->parity ^= 1;
+    g->sweeps_done += 1;
     return PSKEL_OK;
 }
 

@@ -75,6 +75,9 @@ struct Grid {
     size_t bytes = 0;   // per buffer
     float store_offset = 0.0f;
     int device = 0;
+    int rounding = PSKEL_ROUND_NEAREST;
+    uint32_t seed = 0;
+    uint64_t sweeps_done = 0;  // sweeps since the last INPUT upload (stochastic rounding stream)
 
     void* buf[3] = {nullptr, nullptr, nullptr}; // [cur, other, spare]
     int parity = 0;       // which uploaded buffer's ring `cur` carries (0 = input)
@@ -112,6 +115,9 @@ struct Sweep3D {
     int radius;
     float store_offset;
     double* partials;   // non-null: accumulate sum((new-old)^2) per block
+    int rounding;       // pskel_rounding_t of 16-bit stores
+    uint32_t seed;
+    uint64_t sweep;     // index of the sweep this launch performs
 };
 
 // Planned launch shape for a residual-reduction: number of blocks a 3D sweep uses.

@@ -1,5 +1,6 @@
 """Pins the CPU oracle's 3D runners, storage conversions and FP64 MLP."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -155,6 +156,186 @@ def test_r16_stays_close_to_f64():
         got = oracle.decode16(got_w, store).astype(np.float64)
         rel = np.linalg.norm(got - want) / np.linalg.norm(want)
         assert rel < tol, (store, rel)
+
+
+def _bf16_val(w):
+    return float(oracle.decode16(np.array([w], np.uint16), oracle.STORE_BF16)[0])
+
+
+def _f16_val(w):
+    return float(np.array([w], np.uint16).view(np.float16).astype(np.float64)[0])
+
+
+@pytest.mark.parametrize("x", [1.0 + 2 ** -9, -3.14159, 0.3, 1e-20, -7.5e-39, 65504.0, 3.3895e38])
+def test_bf16_stochastic_exact_probability(x):
+    """Over all 2^16 values of the random word's top half, the result rounds the
+    magnitude up exactly `dropped` times (P(up) = dropped fraction of an ulp) and
+    otherwise truncates, so the mean of the decoded results equals x exactly
+    (finite results).  The low half of the word is ignored."""
+    xf = np.float32(x)
+    bits = int(xf.view(np.uint32))
+    dropped = bits & 0xFFFF
+    down = bits >> 16
+    vals = [oracle.encode16_sr(xf, oracle.STORE_BF16, k << 16) for k in range(1 << 16)]
+    assert sum(1 for v in vals if v != down) == dropped
+    assert all(v in (down, down + 1) for v in vals)
+    if all((v & 0x7F80) != 0x7F80 for v in vals):
+        assert sum(_bf16_val(v) for v in vals) / (1 << 16) == float(xf)
+    assert oracle.encode16_sr(xf, oracle.STORE_BF16, 0x1234FFFF) == oracle.encode16_sr(xf, oracle.STORE_BF16, 0x12340000)
+
+
+@pytest.mark.parametrize("x", [1.0 + 2 ** -12, -0.3, 1000.123, 6.2e-5, -2.0 ** -14 * 1.5, 65500.0])
+def test_f16_stochastic_exact_probability_normal(x):
+    """Normal f16 range: the top 13 bits of the random word decide.  Over all
+    2^13 of them the two bracketing f16 values occur with the exact weights that
+    make the mean equal x; a word of 0 truncates."""
+    xf = np.float32(x)
+    vals = [oracle.encode16_sr(xf, oracle.STORE_F16, k << 19) for k in range(1 << 13)]
+    assert len(set(vals)) <= 2
+    assert sum(_f16_val(v) for v in vals) / (1 << 13) == float(xf)
+    t = _f16_val(oracle.encode16_sr(xf, oracle.STORE_F16, 0))
+    assert abs(t) <= abs(float(xf)) < abs(t) + abs(float(xf)) * 2 ** -9
+    assert oracle.encode16_sr(xf, oracle.STORE_F16, (77 << 19) | 0x7FFFF) == oracle.encode16_sr(xf, oracle.STORE_F16, 77 << 19)
+
+
+def test_f16_stochastic_overflow():
+    """Above the largest f16 (65504) the round-up neighbour is 65536, which is inf."""
+    for x, frac in ((65504.0, 0.0), (65510.0, 6 / 32), (65519.0, 15 / 32), (65535.0, 31 / 32)):
+        vals = [oracle.encode16_sr(np.float32(x), oracle.STORE_F16, k << 19) for k in range(1 << 13)]
+        assert set(vals) <= {0x7BFF, 0x7C00}
+        assert sum(v == 0x7C00 for v in vals) == round(frac * 8192)
+
+
+@pytest.mark.parametrize("x", [3.3e-6, -1.0e-7, 2.0 ** -20 + 2.0 ** -30, 5.0e-8, 2.0 ** -14 - 2.0 ** -30, 1e-30, 0.0])
+def test_f16_stochastic_subnormal_mean(x):
+    """Below 2^-14 the f16 spacing is 2^-24: the result is one of the two
+    multiples of 2^-24 around |x|, and over all 2^13 random words its mean is
+    |x| truncated to a multiple of 2^-37 (the dropped fraction counts to 13
+    bits, as in the normal range)."""
+    xf = np.float32(x)
+    vals = [oracle.encode16_sr(xf, oracle.STORE_F16, k << 19) for k in range(1 << 13)]
+    ax = abs(float(xf))
+    lo = np.floor(ax * 2.0 ** 24) * 2.0 ** -24
+    assert {abs(_f16_val(v)) for v in vals} <= {lo, lo + 2.0 ** -24}
+    assert sum(abs(_f16_val(v)) for v in vals) / (1 << 13) == np.floor(ax * 2.0 ** 37) * 2.0 ** -37
+    assert all((v & 0x8000) == (0x8000 if np.signbit(xf) else 0) for v in vals)
+
+
+def test_stochastic_nonfinite():
+    inf, ninf = np.float32(np.inf), np.float32(-np.inf)
+    for r in (0, 0xFFFFFFFF, 0x80000000):
+        assert oracle.encode16_sr(inf, oracle.STORE_BF16, r) == 0x7F80
+        assert oracle.encode16_sr(ninf, oracle.STORE_BF16, r) == 0xFF80
+        assert oracle.encode16_sr(inf, oracle.STORE_F16, r) == 0x7C00
+        assert oracle.encode16_sr(ninf, oracle.STORE_F16, r) == 0xFC00
+        nan = np.float32(np.nan)
+        assert oracle.encode16_sr(nan, oracle.STORE_BF16, r) & 0x7FC0 == 0x7FC0
+        assert oracle.encode16_sr(nan, oracle.STORE_F16, r) & 0x7E00 == 0x7E00
+
+
+def test_stochastic_matches_recorded_gfx950_conversions():
+    """4096 (value, random word) pairs with the results of gfx950's
+    v_cvt_sr_bf16_f32 / v_cvt_sr_f16_f32 recorded on the device
+    (tests/golden/cvt_sr_probe.hip): special values, every bit pattern class,
+    the f16 range and its subnormal band."""
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "cvt_sr_gfx950.npz"))
+    x = f["x_bits"].view(np.float32)
+    r = f["r"]
+    nan = np.isnan(x)
+    for store, want in ((oracle.STORE_BF16, f["bf16"]), (oracle.STORE_F16, f["f16"])):
+        got = np.array([oracle.encode16_sr(v, store, int(k)) for v, k in zip(x, r)], np.uint16)
+        # NaN payloads pass through a double on the ctypes call: compare class only
+        assert np.array_equal(got[~nan], want[~nan]), store
+        q = 0x7FC0 if store == oracle.STORE_BF16 else 0x7E00
+        assert np.all((got[nan] & q) == q) and np.all((want[nan] & q) == q)
+
+
+def test_sr_bits_statistics():
+    """Random words A(w*D + d) ^ B(h): the top 16 bits close to uniform per bit
+    along a plane and along a column; pairwise decorrelated between cells of a
+    plane, cells of a column, consecutive sweeps and seeds.  Two cells of one
+    column in different planes are related within a sweep by the fixed word
+    B(h) ^ B(h'), which is itself uniform over sweeps: over many sweeps their
+    words are uncorrelated."""
+    n = 1 << 14
+    top = lambda a: (a >> np.uint64(16)).astype(np.float64) / 2 ** 16
+    plane = np.array([oracle.sr_bits(11, 0, 5, c) for c in range(n)], np.uint64)
+    column = np.array([oracle.sr_bits(11, 0, h, 77) for h in range(n)], np.uint64)
+    nxt = np.array([oracle.sr_bits(11, 1, 5, c) for c in range(n)], np.uint64)
+    seed2 = np.array([oracle.sr_bits(12, 0, 5, c) for c in range(n)], np.uint64)
+    for a in (plane, column):
+        freq = np.array([((a >> np.uint64(k)) & np.uint64(1)).mean() for k in range(16, 32)])
+        assert np.all(np.abs(freq - 0.5) < 5 * 0.5 / np.sqrt(n))
+    lim = 5 / np.sqrt(n)
+    fp, fc = top(plane), top(column)
+    assert abs(np.corrcoef(fp[:-1], fp[1:])[0, 1]) < lim
+    assert abs(np.corrcoef(fc[:-1], fc[1:])[0, 1]) < lim
+    assert abs(np.corrcoef(fp, top(nxt))[0, 1]) < lim
+    assert abs(np.corrcoef(fp, top(seed2))[0, 1]) < lim
+    h5 = np.array([oracle.sr_bits(11, s, 5, 77) for s in range(n)], np.uint64)
+    h6 = np.array([oracle.sr_bits(11, s, 6, 77) for s in range(n)], np.uint64)
+    assert abs(np.corrcoef(top(h5), top(h6))[0, 1]) < lim
+
+
+def test_r16_stochastic_sweep_is_unbiased():
+    """One sweep of a field whose h-planes alternate between 1 + 2^-8 and 1
+    (exact in both formats at offset 1.0): the interior updates are 1 + 2^-8 *
+    {2/3, 1/3} in f32, inexact in 16 bits.  Nearest rounding errs by the same
+    third of an ulp in every cell; stochastic rounding lands on one of the two
+    neighbours of every f32 value and averages back to it within the sampling
+    error."""
+    n = 40
+    u = np.ones((n, n, n), np.float32)
+    u[0::2] += np.float32(2 ** -8)
+    want = oracle.run_3d(K.jacobi3d(), u, 1)[0][1:-1, 1:-1, 1:-1].astype(np.float64)
+    for store, mbits in ((oracle.STORE_BF16, 7), (oracle.STORE_F16, 10)):
+        w = oracle.encode16(u, store, 1.0)
+        assert np.array_equal(oracle.decode16(w, store, 1.0), u)
+        ulp = 2.0 ** (np.floor(np.log2(want - 1.0)) - mbits)      # per-cell spacing of (v - 1)
+        sigma = 0.5 * np.sqrt(np.mean(ulp ** 2) / want.size)
+        bias = {}
+        for mode in (oracle.ROUND_NEAREST, oracle.ROUND_STOCHASTIC):
+            got_w, _ = oracle.run_3d_r16(K.jacobi3d(), store, w, 1, store_offset=1.0, rounding=mode, seed=3)
+            err = oracle.decode16(got_w, store, 1.0)[1:-1, 1:-1, 1:-1].astype(np.float64) - want
+            assert np.all(np.abs(err) < ulp), (store, mode)
+            bias[mode] = err.mean()
+        assert abs(bias[oracle.ROUND_STOCHASTIC]) < 6 * sigma, (store, bias, sigma)
+        assert abs(bias[oracle.ROUND_NEAREST]) > 20 * sigma, (store, bias, sigma)
+
+
+def test_r16_stochastic_reproducible_and_resumable():
+    u = inputs.jacobi3d_field(16, seed=2)
+    w = oracle.encode16(u, oracle.STORE_BF16, 0.5)
+    full, _ = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, w, 7, store_offset=0.5,
+                                rounding=oracle.ROUND_STOCHASTIC, seed=5)
+    again, _ = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, w, 7, store_offset=0.5,
+                                 rounding=oracle.ROUND_STOCHASTIC, seed=5)
+    assert np.array_equal(full, again)
+    part, _ = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, w, 3, store_offset=0.5,
+                                rounding=oracle.ROUND_STOCHASTIC, seed=5)
+    rest, _ = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, part, 4, store_offset=0.5,
+                                rounding=oracle.ROUND_STOCHASTIC, seed=5, first_sweep=3)
+    assert np.array_equal(rest, full)
+    other, _ = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, w, 7, store_offset=0.5,
+                                 rounding=oracle.ROUND_STOCHASTIC, seed=6)
+    assert not np.array_equal(other, full)
+
+
+def test_r16_stochastic_beats_nearest_vs_f64():
+    """48^3 x 300 sweeps from a uniform field with unit faces: nearest bf16
+    stagnates (error grows with the sweep count), stochastic bf16 stays under
+    1e-3 relative L2 of the FP64 run."""
+    u = inputs.jacobi3d_field(48, seed=9).astype(np.float64)
+    want, _ = oracle.run_3d(K.jacobi3d(), u, 300)
+    rel = {}
+    for mode in (oracle.ROUND_NEAREST, oracle.ROUND_STOCHASTIC):
+        w = oracle.encode16(u.astype(np.float32), oracle.STORE_BF16, 0.5)
+        got_w, _ = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, w, 300, store_offset=0.5,
+                                     rounding=mode, seed=7)
+        got = oracle.decode16(got_w, oracle.STORE_BF16, 0.5).astype(np.float64)
+        rel[mode] = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert rel[oracle.ROUND_STOCHASTIC] < 1e-3, rel
+    assert rel[oracle.ROUND_NEAREST] > 2.5 * rel[oracle.ROUND_STOCHASTIC], rel
 
 
 # ---------------------------------------------------------------------------
