@@ -74,10 +74,11 @@ def field_of(name):
 
 
 def store_offset_of(name):
-    # 16-bit storage holds u - 0.75: the solution relaxes from the interior mean
-    # 0.5 towards the face value 1.0, so deviations from their midpoint stay
-    # small and keep more absolute precision
-    return 0.75 if name in ("jacobi3d_bf16", "jacobi3d_f16") else 0.0
+    # 16-bit storage holds u - 0.5, the mean of the initial field: over 1000
+    # sweeps of a 512^3 grid most cells stay near it (the faces' pull towards
+    # 1.0 reaches a few tens of cells), so the stored deviations are small and
+    # keep more absolute precision
+    return 0.5 if name in ("jacobi3d_bf16", "jacobi3d_f16") else 0.0
 
 
 def rounding_of(name):
@@ -219,8 +220,8 @@ def measure_batch(steps, warmup, sweeps=200):
 
 def accuracy_3d(sweeps=1000, every=50, n=512):
     """Jacobi-3D n^3 x sweeps against the FP64 CPU oracle: f32 storage, and bf16
-    and f16 storage of u - 0.75 with stochastic rounding (f32 arithmetic, tuned
-    kernels), plus three bf16 controls (nearest rounding; offset 0.5; no
+    and f16 storage of u - 0.5 with stochastic rounding (f32 arithmetic, tuned
+    kernels), plus three bf16 controls (nearest rounding; offset 0.75; no
     offset).  Reports the max absolute
     and relative-L2 error of each GPU result and the FP64 residuals the device
     reduces every `every` sweeps next to the oracle's."""
@@ -238,9 +239,9 @@ def accuracy_3d(sweeps=1000, every=50, n=512):
     norm = float(np.linalg.norm(want))
     out = {"oracle": f"FP64 runIterativeCPU restatement, {threads} threads, {oracle_s:.0f} s",
            "grid": f"{n}^3 x {sweeps}", "residual_every": every}
-    for name, storage, off, rnd in (("f32", "f32", 0.0, "nearest"), ("bf16", "bf16", 0.75, "stochastic"),
-                                    ("f16", "f16", 0.75, "stochastic"), ("bf16_nearest", "bf16", 0.75, "nearest"),
-                                    ("bf16_offset_0.5", "bf16", 0.5, "stochastic"),
+    for name, storage, off, rnd in (("f32", "f32", 0.0, "nearest"), ("bf16", "bf16", 0.5, "stochastic"),
+                                    ("f16", "f16", 0.5, "stochastic"), ("bf16_nearest", "bf16", 0.5, "nearest"),
+                                    ("bf16_offset_0.75", "bf16", 0.75, "stochastic"),
                                     ("bf16_no_offset", "bf16", 0.0, "stochastic")):
         kw = dict(TUNED["jacobi3d" if storage == "f32" else "jacobi3d_" + storage])
         with api.Grid(f32.shape, storage, store_offset=off, rounding=rnd, seed=17) as g:
@@ -261,7 +262,7 @@ def accuracy_3d(sweeps=1000, every=50, n=512):
     return out
 
 
-def cpu_baseline(name, seconds_target=12.0):
+def cpu_baseline(name, seconds_target=20.0):
     """The oracle (runIterativeCPU restatement, OpenMP) on a bounded sample:
     the full grid for as many sweeps as fit about `seconds_target`."""
     import oracle
@@ -290,10 +291,20 @@ def cpu_baseline(name, seconds_target=12.0):
     }
 
 
+# rocprofv3 summary of the dominant kernel of each workload (kernel trace + PMC traffic)
+PROFILES = {
+    "jacobi3d": "k3d_march_jacobi3d_512_f32.txt",
+    "jacobi3d_bf16": "k3d_march_jacobi3d_512_bf16_stochastic.txt",
+    "jacobi3d_f16": "k3d_march_jacobi3d_512_f16_stochastic.txt",
+}
+
+
 def traffic_from_profiles(name):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
-    p = os.path.join(ROOT, "profiles", f"traffic_{name}.txt")
+    if name not in PROFILES:
+        return None
+    p = os.path.join(ROOT, "profiles", PROFILES[name])
     if not os.path.exists(p):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read(), re.M)

@@ -139,7 +139,7 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
     }
     if (q.ndim == 3 && pat == PAT_J7) {
         if (q.storage == PSKEL_DTYPE_F32) return tuned(4, 1, 128, 1);
-        return tuned(4, 1, q.storage == PSKEL_DTYPE_BF16 ? 32 : 64, 1);
+        return q.storage == PSKEL_DTYPE_BF16 ? tuned(2, 2, 32, 1) : tuned(4, 2, 32, 1);
     }
     return plain(64, 4);
 }
