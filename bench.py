@@ -40,6 +40,9 @@ WORKLOADS = {
     "life": dict(shape=(8194, 8194), storage="u8", sweeps=500, label="life B3/S23 8192^2 u8 x500"),
     "jacobi2d": dict(shape=(2048, 2048), storage="f32", sweeps=100, label="jacobi2d-5pt 2048^2 f32 x100"),
     "diffusion9": dict(shape=(4096, 4096), storage="f32", sweeps=1000, label="diffusion9 4096^2 f32 x1000"),
+    # field arguments: the grid plus two wind fields read every sweep
+    "cloudsim": dict(shape=(4096, 4096), storage="f32", sweeps=1000, label="cloudsim 4096^2 f32 + 2 wind fields x1000",
+                     bytes_per_cu=16),
 }
 
 # Default tuned launch shapes (overridden by --tile / --time-tile).
@@ -50,6 +53,7 @@ TUNED = {
     "life": dict(tile=(0, 16, 0), time_tile=4),
     "jacobi2d": dict(tile=(0, 16, 0), time_tile=4),
     "diffusion9": dict(tile=(0, 16, 0), time_tile=3),
+    "cloudsim": dict(tile=(0, 1, 2)),
 }
 
 
@@ -58,7 +62,7 @@ def stencil_of(name):
 
     if name.startswith("jacobi3d"):
         return K.jacobi3d()
-    return {"life": K.life, "jacobi2d": K.jacobi2d, "diffusion9": K.diffusion9}[name]()
+    return {"life": K.life, "jacobi2d": K.jacobi2d, "diffusion9": K.diffusion9, "cloudsim": K.cloudsim}[name]()
 
 
 def field_of(name):
@@ -70,6 +74,8 @@ def field_of(name):
         return inputs.life_field(8192, seed=2)
     if name == "jacobi2d":
         return inputs.jacobi2d_field(2048, 2048, seed=1)
+    if name == "cloudsim":
+        return inputs.cloudsim_fields(4096, 4096, seed=6)
     return inputs.blobs_field(4096, seed=3)
 
 
@@ -98,8 +104,14 @@ def measure(name, variant, steps, warmup, field=None, run_kw=None):
         kw = dict(TUNED[name])
     f = field_of(name) if field is None else field
     s = stencil_of(name)
+    winds = []
+    if name == "cloudsim":
+        f, wx, wy = f
+        winds = [api.Field(wx), api.Field(wy)]
     with api.Grid(w["shape"], w["storage"], store_offset=store_offset_of(name), rounding=rounding_of(name),
                   seed=17) as g:
+        for slot, fld in enumerate(winds):
+            g.set_field(slot, fld)
         g.upload_pair(f)
         for _ in range(warmup):
             g.run(s, w["sweeps"], variant=variant, **kw)
@@ -115,7 +127,10 @@ def measure(name, variant, steps, warmup, field=None, run_kw=None):
         wall = time.perf_counter() - t0
     cells = int(np.prod(w["shape"]))
     cu = cells * w["sweeps"] * steps
+    for fld in winds:
+        fld.close()
     elem = {"f32": 4, "bf16": 2, "f16": 2, "u8": 1}[w["storage"]]
+    bpc = w.get("bytes_per_cu", 2 * elem)
     return {
         "workload": w["label"],
         "variant": variant,
@@ -126,8 +141,8 @@ def measure(name, variant, steps, warmup, field=None, run_kw=None):
         "launches_per_step": launches // max(steps, 1),
         "avg_launch_ms": dev_ms / max(launches, 1),
         "sweeps_per_launch": w["sweeps"] * steps / max(launches, 1),
-        "algorithmic_bytes_per_cu": 2 * elem,
-        "hbm_roofline_frac": (cu / wall) * 2 * elem / (HBM_PEAK_GBS * 1e9),
+        "algorithmic_bytes_per_cu": bpc,
+        "hbm_roofline_frac": (cu / wall) * bpc / (HBM_PEAK_GBS * 1e9),
         "cells": cells,
     }
 
@@ -386,7 +401,7 @@ def main(argv=None):
     if not args.no_extra:
         plan = [("jacobi3d", "plain"), ("jacobi3d_bf16", "tuned"), ("jacobi3d_f16", "tuned"),
                 ("life", "tuned"), ("life", "plain"), ("jacobi2d", "tuned"), ("jacobi2d", "plain"),
-                ("diffusion9", "tuned"), ("diffusion9", "plain")]
+                ("diffusion9", "tuned"), ("diffusion9", "plain"), ("cloudsim", "tuned"), ("cloudsim", "plain")]
         for nm, var in plan:
             if (nm, var) == (args.workload, args.variant):
                 continue

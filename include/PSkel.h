@@ -17,6 +17,7 @@
 
 #include "PSkelDefs.h"
 #include "PSkelArray.h"
+#include "PSkelArgs.h"
 #include "PSkelMask.h"
 #include "PSkelStencil.h"
 #include "PSkelStencilTiling.h"

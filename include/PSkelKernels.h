@@ -25,6 +25,7 @@
 #include "PSkelArray.h"
 #include "PSkelDefs.h"
 #include "PSkelMask.h"
+#include "PSkelArgs.h"
 #include "PSkelStencil.h"
 
 namespace PSkel {
@@ -42,6 +43,14 @@ struct WeightedSumArgs {
 
 struct LifeArgs {
     int unused;
+};
+
+// Wind advection + conduction (the reference's CLOUDSIM_KERNEL formula,
+// include/PSkelStencil.hpp:349-387) with the wind as field arguments, read as
+// wind_x(h, w) / wind_y(h, w) like the reference's kernel reads them.
+struct CloudsimArgs {
+    Args2D<float> wind_x, wind_y;
+    float deltaT;
 };
 
 // ---- 2D -----------------------------------------------------------------------
@@ -69,6 +78,31 @@ __parallel__ void stencilKernel(Array2D<T> input, Array2D<T> output, Mask2D<TM> 
     int n = 0;
     for (size_t i = 0; i < mask.size; ++i) n += (int)mask.get(i, input, h, w);
     output(h, w) = (T)((n == 3 || (n == 2 && input(h, w) == 1)) ? 1 : 0);
+}
+
+template <typename TM>
+__parallel__ void stencilKernel(Array2D<float> input, Array2D<float> output, Mask2D<TM> mask, CloudsimArgs args,
+                                size_t h, size_t w)
+{
+    (void)mask;
+    const float c = input(h, w);
+    const float n = input(h - 1, w);
+    const float e = input(h, w + 1);
+    const float ww = input(h, w - 1);
+    const float s = input(h + 1, w);
+    const float sum = 4 * c - (n + e + ww + s);
+    const float xwind = args.wind_x((int)h, (int)w);
+    const float ywind = args.wind_y((int)h, (int)w);
+    const int xfactor = (xwind > 0) ? 1 : -1;
+    const int yfactor = (ywind > 0) ? 1 : -1;
+    const float comp_y = yfactor * ywind;
+    const float comp_x = xfactor * xwind;
+    const float nb_x = input(h, (size_t)((long)w + xfactor));
+    const float nb_y = input((size_t)((long)h + yfactor), w);
+    const float temp_wind = (-comp_x * ((c - nb_x) * 10.0f)) - (comp_y * ((c - nb_y) * 10.0f));
+    const float conduction = -0.0243f * (sum * 0.25f) * args.deltaT;
+    const float result = c + conduction;
+    output(h, w) = result + temp_wind * args.deltaT;
 }
 
 // ---- 3D -----------------------------------------------------------------------
@@ -144,6 +178,24 @@ struct StencilTraits<WeightedSumArgs> {
         pskel_stencil_desc_t d = detail::describeMask(mask, PSKEL_KIND_WEIGHTED_SUM, ndim);
         d.boundary = a.boundary;
         return d;
+    }
+};
+
+template <>
+struct StencilTraits<CloudsimArgs> {
+    static constexpr bool has_descriptor = true;
+    static constexpr int num_fields = 2;  // grid field slots 0, 1 = wind_x, wind_y
+    template <class Mask>
+    static pskel_stencil_desc_t describe(Mask& mask, const CloudsimArgs& a, int ndim)
+    {
+        pskel_stencil_desc_t d = detail::describeMask(mask, PSKEL_KIND_CLOUDSIM, ndim);
+        d.coeff = a.deltaT;
+        return d;
+    }
+    static void fields(const CloudsimArgs& a, pskel_field_t** out)
+    {
+        out[0] = a.wind_x.deviceField();
+        out[1] = a.wind_y.deviceField();
     }
 };
 

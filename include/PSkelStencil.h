@@ -44,6 +44,7 @@
 #endif
 
 #include "PSkelArray.h"
+#include "PSkelArgs.h"
 #include "PSkelDefs.h"
 #include "PSkelMask.h"
 
@@ -62,6 +63,12 @@ struct StencilTraits {
 };
 
 namespace detail {
+
+// StencilTraits that name field arguments (num_fields, fields(args, out))
+template <class Tr, class = void>
+struct hasFields : std::false_type {};
+template <class Tr>
+struct hasFields<Tr, std::void_t<decltype(Tr::num_fields)>> : std::true_type {};
 
 inline size_t cpuThreads(size_t n)
 {
@@ -352,8 +359,16 @@ protected:
         cfg.residual_every = (int)s.residualEvery;
         s.residuals.clear();
 
+        // field arguments (Args/Args2D/Args3D): their host arrays go to HBM once per call
+        detail::syncFields();
         if constexpr (StencilTraits<Args>::has_descriptor) {
             pskel_stencil_desc_t desc = StencilTraits<Args>::describe(mask, args, dims());
+            if constexpr (detail::hasFields<StencilTraits<Args>>::value) {
+                pskel_field_t* fl[StencilTraits<Args>::num_fields];
+                StencilTraits<Args>::fields(args, fl);
+                for (int k = 0; k < StencilTraits<Args>::num_fields; ++k)
+                    gpuErrchk(pskel_grid_set_field(s.grid, k, fl[k]));
+            }
             if (autotune && !s.haveCfg) {
                 pskel_tune_query_t q{};
                 q.ndim = dims();

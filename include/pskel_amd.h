@@ -50,12 +50,22 @@ typedef enum pskel_dtype {
  *   SCALED_SUM   out = coeff * (((p0 + p1) + p2) + ...)
  *   WEIGHTED_SUM out = ((w0*p0 + w1*p1) + w2*p2) + ...     (w_i = Mask weight i)
  *   LIFE         n = p0 + p1 + ...; out = (n == 3 || (n == 2 && centre == 1))
+ *   CLOUDSIM     2D wind advection + conduction with two field arguments, the
+ *                CLOUDSIM_KERNEL formula (include/PSkelStencil.hpp:349-387) on
+ *                the global neighbours of every interior cell (h, w):
+ *                  c = in(h,w); n = in(h-1,w); e = in(h,w+1); w_ = in(h,w-1); s = in(h+1,w)
+ *                  sum = 4*c - (((n + e) + w_) + s)
+ *                  xw = wind_x(h,w); yw = wind_y(h,w)   (field slots 0 and 1)
+ *                  xf = xw > 0 ? 1 : -1; yf = yw > 0 ? 1 : -1
+ *                  wind = (-(xf*xw) * ((c - in(h,w+xf)) * 10)) - ((yf*yw) * ((c - in(h+yf,w)) * 10))
+ *                  out = (c + (-0.0243 * (sum * 0.25)) * dt) + wind * dt     (dt = coeff)
  * Each product and sum rounds on its own (no fused multiply-add), so results
  * are bit-identical to runIterativeCPU built the same way. */
 typedef enum pskel_kind {
     PSKEL_KIND_SCALED_SUM = 1,
     PSKEL_KIND_WEIGHTED_SUM = 2,
-    PSKEL_KIND_LIFE = 3
+    PSKEL_KIND_LIFE = 3,
+    PSKEL_KIND_CLOUDSIM = 4
 } pskel_kind_t;
 
 /* 3D boundary rule (Stencil3D visits every cell, include/PSkelStencil.hpp:2095-2113):
@@ -154,7 +164,8 @@ typedef struct pskel_grid_info {
 /* ---- library ------------------------------------------------------------ */
 int32_t pskel_abi_version(void);
 /* sizeof of the ABI structs, for bindings to verify their layout:
- * 0 stencil_desc, 1 launch_cfg, 2 run_stats, 3 grid_desc, 4 grid_info; else 0. */
+ * 0 stencil_desc, 1 launch_cfg, 2 run_stats, 3 grid_desc, 4 grid_info,
+ * 5 field_desc; else 0. */
 size_t pskel_abi_struct_size(int32_t which);
 const char* pskel_status_string(pskel_status_t st);
 /* Message of the last failure on the calling thread ("" if none). */
@@ -207,6 +218,42 @@ pskel_status_t pskel_grid_swap(pskel_grid_t* grid);
 /* FP64 ||current - previous||_2 over every cell (the convergence test the
  * reference leaves to the caller). */
 pskel_status_t pskel_grid_residual(pskel_grid_t* grid, double* out);
+
+/* ---- field arguments: Args / Args2D / Args3D (include/PSkelArgs.h:31-132,
+ *      include/PSkelArgs.hpp:49-161), device-resident instead of the
+ *      reference's zero-copy mapped host memory, so sweeps read them from HBM.
+ *      A field has the Args extents (width, height, depth) = (x, y, z) ranges;
+ *      its host data is in the Args layout, x fastest: element (x, y, z) at
+ *      (z*height + y)*width + x (include/PSkelArgs.hpp:116-123, 155-160).  On
+ *      the device the axes are reversed so that a kernel reading args.f(h, w)
+ *      or args.f(h, w, d), as the reference's kernels call them (x = h,
+ *      y = w, z = d; include/PSkelStencil.hpp:366-369), reads along the grid's
+ *      contiguous axis: element (x, y) at x*pitch + y, element (x, y, z) at
+ *      (x*height + y)*pitch + z, element x of a 1D field at x. */
+typedef struct pskel_field pskel_field_t;
+
+typedef struct pskel_field_desc {
+    int32_t ndim;        /* 1, 2 or 3: Args, Args2D, Args3D */
+    int32_t dtype;       /* PSKEL_DTYPE_F32 */
+    size_t width;        /* x extent */
+    size_t height;       /* y extent (1 for 1D) */
+    size_t depth;        /* z extent (1 for 1D and 2D) */
+    int32_t device;      /* HIP device ordinal */
+    int32_t reserved[3];
+} pskel_field_desc_t;
+
+pskel_status_t pskel_field_create(const pskel_field_desc_t* desc, pskel_field_t** out);
+pskel_status_t pskel_field_destroy(pskel_field_t* field);
+/* host -> device, host in the Args layout (x fastest) */
+pskel_status_t pskel_field_upload(pskel_field_t* field, const void* host);
+/* device -> host, into the Args layout */
+pskel_status_t pskel_field_download(const pskel_field_t* field, void* host);
+/* device base pointer and pitch (elements) of the reversed layout above */
+pskel_status_t pskel_field_info(const pskel_field_t* field, void** device_ptr, size_t* pitch);
+/* Attach a field to slot 0..3 of a grid (not owned; NULL detaches).  CLOUDSIM
+ * reads wind_x from slot 0 and wind_y from slot 1; both must be 2D fields with
+ * width >= grid height and height >= grid width. */
+pskel_status_t pskel_grid_set_field(pskel_grid_t* grid, int32_t slot, pskel_field_t* field);
 
 /* ---- batched persistent sweeps: many uneven 2D grids advanced together by
  *      one persistent kernel per sweep (a fixed set of blocks walking a list

@@ -24,6 +24,9 @@ function (see tests/test_oracle_*.py):
                    equality with the f32 runner on exactly representable data;
                    stochastic rounding: exhaustive probability counts, closed
                    forms, recorded gfx950 v_cvt_sr outputs, unbiasedness
+  run_2d_cloud     zero-wind conduction eigenmode decay and fixed points, pure
+                   advection of a linear ramp (closed form), upwind side chosen
+                   by the wind sign, the Args2D (x, y) -> y*width + x field map
   mlp              finite differences of the loss (central, FP64)
 """
 from __future__ import annotations
@@ -89,6 +92,8 @@ def lib():
         l.oracle_f32_to_bf16_sr.restype = C.c_uint16
         l.oracle_f32_to_f16_sr.argtypes = [C.c_float, C.c_uint32]
         l.oracle_f32_to_f16_sr.restype = C.c_uint16
+        l.oracle_run_iterative_2d_cloud.argtypes = [P, P, SZ, SZ, P, P, SZ, C.c_float, SZ, I]
+        l.oracle_run_iterative_2d_cloud.restype = None
         l.oracle_range.argtypes = [D, I]
         l.oracle_range.restype = C.c_int32
         l.oracle_f32_to_bf16.argtypes = [C.c_float]
@@ -165,6 +170,22 @@ def run_3d(stencil, inp: np.ndarray, iterations: int, out: np.ndarray | None = N
     fn = l.oracle_run_iterative_3d_f32 if a.dtype == np.float32 else l.oracle_run_iterative_3d_f64
     n = fn(C.byref(d), _ptr(a), _ptr(b), W, H, D, int(iterations), int(threads), int(residual_every), _ptr(res))
     return b, res[:n]
+
+
+def run_2d_cloud(inp: np.ndarray, wind_x: np.ndarray, wind_y: np.ndarray, delta_t: float, iterations: int,
+                 out: np.ndarray | None = None, threads: int = 0) -> np.ndarray:
+    """Result of runIterativeCPU with the CLOUDSIM kernel on an (H, W) f32 grid.
+    wind_x / wind_y are Args2D host arrays, shape (field_height, field_width),
+    indexed [y, x]; the kernel reads wind(x=h, y=w)."""
+    a = np.array(inp, dtype=np.float32, copy=True, order="C")
+    b = np.array(inp if out is None else out, dtype=np.float32, copy=True, order="C")
+    wx = np.ascontiguousarray(wind_x, np.float32)
+    wy = np.ascontiguousarray(wind_y, np.float32)
+    H, W = a.shape
+    assert wx.shape == wy.shape and wx.shape[1] >= H and wx.shape[0] >= W
+    lib().oracle_run_iterative_2d_cloud(_ptr(a), _ptr(b), W, H, _ptr(wx), _ptr(wy), wx.shape[1],
+                                        float(np.float32(delta_t)), int(iterations), int(threads))
+    return b
 
 
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1

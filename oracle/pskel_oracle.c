@@ -246,6 +246,56 @@ DEFINE_RUN_2D(oracle_run_iterative_2d_f64, double, sweep2d_f64)
 DEFINE_RUN_2D(oracle_run_iterative_2d_u8, uint8_t, sweep2d_u8)
 
 /* ------------------------------------------------------------------------ */
+/* CLOUDSIM: the wind advection + conduction formula of the reference's       */
+/* CLOUDSIM_KERNEL (include/PSkelStencil.hpp:349-387), evaluated on the       */
+/* global neighbours of every interior cell in the same order.  The wind is   */
+/* read as args.wind_x(h, w), i.e. through Args2D's map (x, y) -> y*width + x  */
+/* (include/PSkelArgs.hpp:116-123) with x = h and y = w: wind[w*fw + h].      */
+/* ------------------------------------------------------------------------ */
+static void sweep2d_cloud(const float* in, float* out, size_t W, size_t H, const float* wx, const float* wy,
+                          size_t fw, float dt, int th)
+{
+    if (H < 2 || W < 2) return;
+#pragma omp parallel for num_threads(th) schedule(static)
+    for (size_t h = 1; h < H - 1; ++h) {
+        for (size_t w = 1; w < W - 1; ++w) {
+            const float c = in[h * W + w];
+            const float n = in[(h - 1) * W + w];
+            const float e = in[h * W + w + 1];
+            const float ww = in[h * W + w - 1];
+            const float s = in[(h + 1) * W + w];
+            const float sum = 4 * c - (n + e + ww + s);
+            const float xwind = wx[w * fw + h];
+            const float ywind = wy[w * fw + h];
+            const int xfactor = (xwind > 0) ? 1 : -1;
+            const int yfactor = (ywind > 0) ? 1 : -1;
+            const float comp_y = yfactor * ywind;
+            const float comp_x = xfactor * xwind;
+            const float nb_x = in[h * W + (size_t)((ptrdiff_t)w + xfactor)];
+            const float nb_y = in[(size_t)((ptrdiff_t)h + yfactor) * W + w];
+            const float temp_wind = (-comp_x * ((c - nb_x) * 10.0f)) - (comp_y * ((c - nb_y) * 10.0f));
+            const float conduction = -0.0243f * (sum * 0.25f) * dt;
+            const float result = c + conduction;
+            out[h * W + w] = result + temp_wind * dt;
+        }
+    }
+}
+
+void oracle_run_iterative_2d_cloud(float* input, float* output, size_t width, size_t height,
+                                   const float* wind_x, const float* wind_y, size_t field_width, float delta_t,
+                                   size_t iterations, int threads)
+{
+    const int th = nthreads(threads);
+    double t0 = omp_get_wtime();
+    for (size_t it = 0; it < iterations; ++it) {
+        if (it % 2 == 0) sweep2d_cloud(input, output, width, height, wind_x, wind_y, field_width, delta_t, th);
+        else sweep2d_cloud(output, input, width, height, wind_x, wind_y, field_width, delta_t, th);
+    }
+    if (iterations % 2 == 0) memcpy(output, input, width * height * sizeof(float));
+    g_last_seconds = omp_get_wtime() - t0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* 3D: Stencil3D::runOpenMP visits every cell (PSkelStencil.hpp:2104-2113)   */
 /* ------------------------------------------------------------------------ */
 typedef struct {

@@ -94,6 +94,12 @@ void oracle_run_iterative_2d_u8(const oracle_desc* d, uint8_t* input, uint8_t* o
 /* runIterativeCPU on 3D grids.  residuals (may be NULL) receives the FP64 norm
  * ||u^k - u^{k-1}||_2 after every sweep k that is a multiple of residual_every
  * (0 disables); the number written is returned. */
+/* CLOUDSIM (2D, f32): wind fields in the Args2D host layout, read at
+ * wind[w*field_width + h]; interior cells only, as every 2D runner. */
+void oracle_run_iterative_2d_cloud(float* input, float* output, size_t width, size_t height,
+                                   const float* wind_x, const float* wind_y, size_t field_width, float delta_t,
+                                   size_t iterations, int threads);
+
 size_t oracle_run_iterative_3d_f32(const oracle_desc* d, float* input, float* output,
                                    size_t width, size_t height, size_t depth, size_t iterations,
                                    int threads, size_t residual_every, double* residuals);

@@ -32,6 +32,7 @@ DTYPE_F16 = 5
 KIND_SCALED_SUM = 1
 KIND_WEIGHTED_SUM = 2
 KIND_LIFE = 3
+KIND_CLOUDSIM = 4
 
 BOUNDARY_COPY = 0
 BOUNDARY_HALO = 1
@@ -129,10 +130,22 @@ class TuneQuery(C.Structure):
     ]
 
 
+class FieldDesc(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("dtype", C.c_int32),
+        ("width", C.c_size_t),
+        ("height", C.c_size_t),
+        ("depth", C.c_size_t),
+        ("device", C.c_int32),
+        ("reserved", C.c_int32 * 3),
+    ]
+
+
 TUNER_MAX_CANDIDATES = 256
 TUNER_NFEAT = 24
 
-STRUCTS = [StencilDesc, LaunchCfg, RunStats, GridDesc, GridInfo]
+STRUCTS = [StencilDesc, LaunchCfg, RunStats, GridDesc, GridInfo, FieldDesc]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpskel_amd.so")
 
@@ -171,6 +184,12 @@ def _declare(lib):
     _sig(lib, "pskel_run_iterative", st, P, C.POINTER(StencilDesc), SZ, C.POINTER(LaunchCfg), C.POINTER(RunStats))
     _sig(lib, "pskel_grid_residual", st, P, C.POINTER(D))
     _sig(lib, "pskel_grid_swap", st, P)
+    _sig(lib, "pskel_grid_set_field", st, P, I32, P)
+    _sig(lib, "pskel_field_create", st, C.POINTER(FieldDesc), C.POINTER(P))
+    _sig(lib, "pskel_field_destroy", st, P)
+    _sig(lib, "pskel_field_upload", st, P, P)
+    _sig(lib, "pskel_field_download", st, P, P)
+    _sig(lib, "pskel_field_info", st, P, C.POINTER(P), C.POINTER(SZ))
     F = C.POINTER(C.c_float)
     _sig(lib, "pskel_batch_create", st, C.POINTER(P), C.POINTER(StencilDesc), C.POINTER(LaunchCfg), SZ, C.POINTER(P))
     _sig(lib, "pskel_batch_run", st, P, SZ, I32, C.POINTER(RunStats))

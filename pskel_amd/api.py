@@ -148,6 +148,61 @@ class Grid:
         L.check(L.lib().pskel_grid_residual(self._h, C.byref(v)))
         return v.value
 
+    def set_field(self, slot: int, field: Optional["Field"]):
+        """Attach a field argument (not owned; keep it alive while runs use it)."""
+        L.check(L.lib().pskel_grid_set_field(self._h, int(slot), field._h if field is not None else None))
+
+
+class Field:
+    """A device-resident Args / Args2D / Args3D field (include/PSkelArgs.h:31-132).
+
+    `host` is the Args host array in the reference's layout, x fastest: a numpy
+    array of shape (width,), (height, width) or (depth, height, width), so
+    host[y, x] is Args2D(x, y) (include/PSkelArgs.hpp:116-123, 155-160)."""
+
+    def __init__(self, host: np.ndarray, device: int = 0):
+        host = np.ascontiguousarray(host, np.float32)
+        fd = L.FieldDesc()
+        fd.ndim = host.ndim
+        fd.dtype = L.DTYPE_F32
+        fd.width = host.shape[-1]
+        fd.height = host.shape[-2] if host.ndim >= 2 else 1
+        fd.depth = host.shape[0] if host.ndim == 3 else 1
+        fd.device = device
+        h = C.c_void_p()
+        L.check(L.lib().pskel_field_create(C.byref(fd), C.byref(h)))
+        self._h = h
+        self.shape = host.shape
+        self.upload(host)
+
+    def upload(self, host: np.ndarray):
+        host = np.ascontiguousarray(host, np.float32)
+        if host.shape != self.shape:
+            raise ValueError(f"field shape {host.shape} != {self.shape}")
+        L.check(L.lib().pskel_field_upload(self._h, host.ctypes.data))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, np.float32)
+        L.check(L.lib().pskel_field_download(self._h, out.ctypes.data))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().pskel_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
 
 class Batch:
     """Many 2D grids swept together by one persistent kernel per sweep

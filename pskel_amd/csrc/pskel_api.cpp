@@ -146,6 +146,7 @@ int classify(const pskel_stencil_desc_t& d, int ndim)
             for (int i = 0; i < 9 && ok; ++i) ok = o[2 * i] == i / 3 - 1 && o[2 * i + 1] == i % 3 - 1;
             if (ok) return PAT_BOX9;
         }
+        if (d.kind == PSKEL_KIND_CLOUDSIM) return PAT_CLOUD;
         if (d.kind == PSKEL_KIND_LIFE && d.npoints == 8) {
             int seen = 0;
             for (int i = 0; i < 8; ++i) {
@@ -380,7 +381,16 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
             Sweep2D s = make_sweep2d(g, src, dst, g->buf[1], plan.radius);
             const int bx = cfg.block_x > 0 ? cfg.block_x : 64;
             const int by = cfg.block_y > 0 ? cfg.block_y : 4;
-            if (g->storage == PSKEL_DTYPE_U8) {
+            if (plan.pattern == PAT_CLOUD) {
+                CloudArgs ca;
+                int fd;
+                size_t fw, fh, fz;
+                field_info(g->fields[0], &fd, &fw, &fh, &fz, &ca.wx, &ca.fpitch);
+                field_info(g->fields[1], &fd, &fw, &fh, &fz, &ca.wy, &ca.fpitch);
+                ca.dt = d.coeff;
+                if (plan.path >= 1) e = launch_cloud_tuned(s, ca, one, st);
+                else e = launch_cloud_plain(s, ca, bx, by, st);
+            } else if (g->storage == PSKEL_DTYPE_U8) {
                 if (k > 1) e = launch_life_ttile(s, (int)k, cfg, st);
                 else if (plan.path >= 1) e = launch_life_tuned(s, one, st);
                 else e = launch_life_plain(s, pts, bx, by, st);
@@ -442,6 +452,7 @@ size_t pskel_abi_struct_size(int32_t which)
     case 2: return sizeof(pskel_run_stats_t);
     case 3: return sizeof(pskel_grid_desc_t);
     case 4: return sizeof(pskel_grid_info_t);
+    case 5: return sizeof(pskel_field_desc_t);
     default: return 0;
     }
 }
@@ -676,8 +687,25 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
     const int ndim = g->ndim;
     if (d->npoints < 1 || d->npoints > PSKEL_MAX_POINTS)
         return fail(PSKEL_ERR_INVALID_ARGUMENT, "npoints must be in [1, 64]");
-    if (d->kind != PSKEL_KIND_SCALED_SUM && d->kind != PSKEL_KIND_WEIGHTED_SUM && d->kind != PSKEL_KIND_LIFE)
+    if (d->kind != PSKEL_KIND_SCALED_SUM && d->kind != PSKEL_KIND_WEIGHTED_SUM && d->kind != PSKEL_KIND_LIFE &&
+        d->kind != PSKEL_KIND_CLOUDSIM)
         return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown stencil kind");
+    if (d->kind == PSKEL_KIND_CLOUDSIM) {
+        if (ndim != 2 || g->storage != PSKEL_DTYPE_F32)
+            return fail(PSKEL_ERR_UNSUPPORTED, "CLOUDSIM runs on 2D f32 grids");
+        for (int k = 0; k < 2; ++k) {
+            int fd;
+            size_t fw, fh, fz, fp;
+            const float* data;
+            if (!field_info(g->fields[k], &fd, &fw, &fh, &fz, &data, &fp))
+                return fail(PSKEL_ERR_INVALID_ARGUMENT, "CLOUDSIM needs wind fields in slots 0 and 1");
+            // the kernel reads field (x = h, y = w) for every interior cell
+            if (fd != 2 || fw < g->H || fh < g->W)
+                return fail(PSKEL_ERR_INVALID_ARGUMENT, "wind fields must be 2D with width >= grid height "
+                                                        "and height >= grid width");
+        }
+        if (desc_radius(*d, 2) != 1) return fail(PSKEL_ERR_INVALID_ARGUMENT, "CLOUDSIM has mask range 1");
+    }
     if ((d->kind == PSKEL_KIND_LIFE) != (g->storage == PSKEL_DTYPE_U8))
         return fail(PSKEL_ERR_UNSUPPORTED, "life runs on u8 grids and u8 grids run life");
     if (ndim == 3 && d->boundary != PSKEL_BOUNDARY_COPY && d->boundary != PSKEL_BOUNDARY_HALO)
@@ -702,10 +730,11 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
     plan.path = 0;
     if (cfg.variant == PSKEL_VARIANT_TUNED) {
         const bool tunable = (ndim == 2 && (plan.pattern == PAT_J5 || plan.pattern == PAT_BOX9 ||
-                                            plan.pattern == PAT_LIFE8)) ||
+                                            plan.pattern == PAT_LIFE8 || plan.pattern == PAT_CLOUD)) ||
                              (ndim == 3 && plan.pattern == PAT_J7);
         if (tunable) plan.path = 1;
-        if (tunable && ndim == 2 && cfg.time_tile > 1) {
+        // field stencils run one sweep per launch (no temporal tiles)
+        if (tunable && ndim == 2 && cfg.time_tile > 1 && plan.pattern != PAT_CLOUD) {
             if (cfg.time_tile > 4) return fail(PSKEL_ERR_UNSUPPORTED, "temporal depth above 4");
             if (cfg.residual_every > 0 && cfg.residual_every < cfg.time_tile)
                 return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
@@ -767,6 +796,7 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
         put(ptrs, sizeof(ptrs));
         put(&sp, sizeof(sp));
         put(&sweep0, sizeof(sweep0));
+        put(g->fields, sizeof(g->fields));
         GraphEntry* hit = nullptr;
         for (auto& ge : g->graphs)
             if (ge.key == key) hit = &ge;
@@ -842,6 +872,15 @@ pskel_status_t pskel_grid_swap(pskel_grid_t* g)
     std::swap(g->buf[0], g->buf[1]);
     g->parity ^= 1;
     g->sweeps_done += 1;
+    return PSKEL_OK;
+}
+
+pskel_status_t pskel_grid_set_field(pskel_grid_t* g, int32_t slot, pskel_field_t* f)
+{
+    pskel_status_t st = check_grid(g);
+    if (st != PSKEL_OK) return st;
+    if (slot < 0 || slot >= 4) return fail(PSKEL_ERR_INVALID_ARGUMENT, "field slot must be 0..3");
+    g->fields[slot] = f;
     return PSKEL_OK;
 }
 

@@ -53,6 +53,7 @@ enum Pattern : int {
     PAT_BOX9 = 2, // weighted sum over the 3x3 box, row-major
     PAT_LIFE8 = 3,// life over the 8 Moore neighbours
     PAT_J7 = 4,   // 3D scaled sum (-1,0,0) (1,0,0) (0,-1,0) (0,1,0) (0,0,-1) (0,0,1)
+    PAT_CLOUD = 5,// 2D CLOUDSIM (wind advection + conduction, two field arguments)
 };
 
 int classify(const pskel_stencil_desc_t& d, int ndim);
@@ -91,6 +92,8 @@ struct Grid {
     size_t partials_cap = 0;
     double* d_res = nullptr;        // reduced residuals, one per residual sweep
     size_t res_cap = 0;
+
+    pskel_field_t* fields[4] = {nullptr, nullptr, nullptr, nullptr};  // attached field arguments
 };
 
 void advance_state(Grid& g, size_t k);
@@ -119,6 +122,20 @@ struct Sweep3D {
     uint32_t seed;
     uint64_t sweep;     // index of the sweep this launch performs
 };
+
+// CLOUDSIM field arguments as its kernels see them (reversed device layout:
+// the field value of cell (h, w) at h*fpitch + w).
+struct CloudArgs {
+    const float* wx;
+    const float* wy;
+    size_t fpitch;
+    float dt;
+};
+
+hipError_t launch_cloud_plain(const Sweep2D& s, const CloudArgs& a, int bx, int by, hipStream_t st);
+hipError_t launch_cloud_tuned(const Sweep2D& s, const CloudArgs& a, const pskel_launch_cfg_t& cfg, hipStream_t st);
+bool field_info(const pskel_field_t* f, int* ndim, size_t* W, size_t* H, size_t* D, const float** data,
+                size_t* pitch);
 
 // Planned launch shape for a residual-reduction: number of blocks a 3D sweep uses.
 size_t sweep3d_blocks(const Grid& g, int pattern, const pskel_launch_cfg_t& cfg);

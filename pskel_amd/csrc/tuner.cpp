@@ -68,11 +68,12 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
 {
     std::vector<pskel_launch_cfg_t> v;
     const int pat = classify(d, q.ndim);
-    if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9 || pat == PAT_LIFE8)) {
+    if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9 || pat == PAT_LIFE8 || pat == PAT_CLOUD)) {
         const int ry_nwy[][2] = {{1, 2}, {1, 4}, {2, 2}, {2, 4}, {4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}};
         for (auto& t : ry_nwy) v.push_back(tuned(0, t[0], t[1], 1));
-        for (int k = 2; k <= 4; ++k)
-            for (int cy : {16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
+        if (pat != PAT_CLOUD)  // field stencils run one sweep per launch
+            for (int k = 2; k <= 4; ++k)
+                for (int cy : {16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
         v.push_back(plain(64, 4));
     } else if (q.ndim == 3 && pat == PAT_J7) {
         const bool narrow = q.storage != PSKEL_DTYPE_F32;
@@ -132,6 +133,7 @@ void features(const pskel_tune_query_t& q, const pskel_stencil_desc_t& d, const 
 pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_desc_t& d)
 {
     const int pat = classify(d, q.ndim);
+    if (q.ndim == 2 && pat == PAT_CLOUD) return tuned(0, 2, 4, 1);
     if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9 || pat == PAT_LIFE8)) {
         const double cells = (double)q.width * q.height;
         if (q.iterations >= 4 && cells >= (1 << 20)) return tuned(0, 16, 0, pat == PAT_BOX9 ? 3 : 4);

@@ -111,3 +111,22 @@ def batch_shapes(count: int = 256, seed: int = 5, lo: int = 64, hi: int = 2048) 
         w = int(round(math.exp(a + (b - a) * u[i, 1])))
         out.append((h, w, "jacobi" if i % 2 == 0 else "life"))
     return out
+
+
+def cloudsim_fields(height: int, width: int, seed: int = 6):
+    """CLOUDSIM inputs: an (height, width) temperature field (interior uniform
+    in [15, 25), ring fixed at 20) and the wind components as Args2D host
+    arrays of shape (width, height), indexed [y, x] with x = grid row h and
+    y = grid column w (the map the kernel reads them through).  The wind is a
+    smooth swirl of magnitude <= 1 plus a small uniform perturbation, so both
+    signs occur (and with them both upwind sides)."""
+    t = (np.float32(15.0) + np.float32(10.0) * uniform(seed, 0, height * width)).reshape(height, width)
+    t[0, :] = t[-1, :] = 20.0
+    t[:, 0] = t[:, -1] = 20.0
+    h = np.arange(height, dtype=np.float64)[None, :]   # x = h
+    w = np.arange(width, dtype=np.float64)[:, None]    # y = w
+    jitter_x = uniform(seed + 1, 0, width * height).reshape(width, height).astype(np.float64) - 0.5
+    jitter_y = uniform(seed + 2, 0, width * height).reshape(width, height).astype(np.float64) - 0.5
+    wind_x = (0.8 * np.sin(2 * math.pi * w / width) + 0.2 * jitter_x).astype(np.float32)
+    wind_y = (0.8 * np.cos(2 * math.pi * h / height) + 0.2 * jitter_y).astype(np.float32)
+    return t.astype(np.float32), wind_x, wind_y

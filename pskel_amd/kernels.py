@@ -27,6 +27,7 @@ from . import _lib as L
 SCALED_SUM = L.KIND_SCALED_SUM
 WEIGHTED_SUM = L.KIND_WEIGHTED_SUM
 LIFE = L.KIND_LIFE
+CLOUDSIM = L.KIND_CLOUDSIM
 
 BOUNDARY_COPY = L.BOUNDARY_COPY
 BOUNDARY_HALO = L.BOUNDARY_HALO
@@ -130,6 +131,13 @@ def diffusion9(centre=0.5, edge=0.1, corner=0.025, wind=(0.02, -0.01)) -> Stenci
     s = weighted9(diffusion9_weights(centre, edge, corner, wind))
     s.name = "diffusion9"
     return s
+
+
+def cloudsim(delta_t: float = 0.01) -> Stencil:
+    """Wind advection + conduction (the reference's CLOUDSIM_KERNEL formula,
+    include/PSkelStencil.hpp:349-387) with wind_x / wind_y field arguments in
+    grid field slots 0 and 1; coeff carries deltaT."""
+    return Stencil(CLOUDSIM, 2, list(J5_OFFSETS), coeff=delta_t, name="cloudsim")
 
 
 def life() -> Stencil:

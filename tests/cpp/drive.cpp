@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "PSkel.h"
@@ -45,6 +46,28 @@ __parallel__ void stencilKernel(Array3D<float> in, Array3D<float> out, Mask3D<fl
     acc = acc + in(h, w, d - 1);
     acc = acc + in(h, w, d + 1);
     out(h, w, d) = a.c * acc;
+}
+// A user kernel reading field arguments of its own Args type (no descriptor):
+// on the GPU it runs only when this file is compiled by hipcc, with the fields
+// read from their device copies.
+struct UserCloudArgs {
+    Args2D<float> wx, wy;
+    float dt;
+};
+
+__parallel__ void stencilKernel(Array2D<float> in, Array2D<float> out, Mask2D<float> mask, UserCloudArgs a, size_t h,
+                                size_t w)
+{
+    (void)mask;
+    const float c = in(h, w);
+    const float sum = 4 * c - (in(h - 1, w) + in(h, w + 1) + in(h, w - 1) + in(h + 1, w));
+    const float xwind = a.wx((int)h, (int)w), ywind = a.wy((int)h, (int)w);
+    const int xf = xwind > 0 ? 1 : -1, yf = ywind > 0 ? 1 : -1;
+    const float cy = yf * ywind, cx = xf * xwind;
+    const float tw = (-cx * ((c - in(h, (size_t)((long)w + xf))) * 10.0f)) -
+                     (cy * ((c - in((size_t)((long)h + yf), w)) * 10.0f));
+    const float result = c + -0.0243f * (sum * 0.25f) * a.dt;
+    out(h, w) = result + tw * a.dt;
 }
 }  // namespace PSkel
 
@@ -116,10 +139,33 @@ static int run3d(Mask mask, Args args, const std::string& method, size_t W, size
     return 0;
 }
 
+// Wind fields for the CLOUDSIM kernels: Args2D(width = H, height = W), host
+// layout from the files (x fastest), read as wind(h, w).
+template <class A>
+static int runCloud(const std::string& method, size_t W, size_t H, size_t iters, char** files, char** wind)
+{
+    A a;
+    Args2D<float> wx((int)H, (int)W), wy((int)H, (int)W);
+    if (!readf(wind[0], wx.hostArray, W * H) || !readf(wind[1], wy.hostArray, W * H)) return 3;
+    if constexpr (std::is_same_v<A, CloudsimArgs>) {
+        a.wind_x = wx;
+        a.wind_y = wy;
+        a.deltaT = 0.01f;
+    } else {
+        a.wx = wx;
+        a.wy = wy;
+        a.dt = 0.01f;
+    }
+    const int rc = run2d<float>(jacobiMask2D<float>(), a, method, W, H, iters, files);
+    wx.hostFree();
+    wy.hostFree();
+    return rc;
+}
+
 int main(int argc, char** argv)
 {
-    if (argc != 10) {
-        std::fprintf(stderr, "usage: drive kernel method W H D iters input output0 result\n");
+    if (argc != 10 && argc != 12) {
+        std::fprintf(stderr, "usage: drive kernel method W H D iters input output0 result [wind_x wind_y]\n");
         return 2;
     }
     const std::string kernel = argv[1], method = argv[2];
@@ -128,6 +174,11 @@ int main(int argc, char** argv)
     char** files = argv + 7;
     try {
         if (kernel == "jacobi2d") return run2d<float>(jacobiMask2D<float>(), ScaledSumArgs{0.25, 0}, method, W, H, iters, files);
+        if (kernel == "cloudsim" || kernel == "usercloud") {
+            if (argc != 12) return 2;
+            if (kernel == "cloudsim") return runCloud<CloudsimArgs>(method, W, H, iters, files, argv + 10);
+            return runCloud<UserCloudArgs>(method, W, H, iters, files, argv + 10);
+        }
         if (kernel == "user2d") return run2d<float>(jacobiMask2D<float>(), UserJacobiArgs{0.25f}, method, W, H, iters, files);
         if (kernel == "diffusion9") {
             // centre 0.5, edge 0.1, corner 0.025, skewed by the wind (0.02, -0.01)

@@ -16,9 +16,10 @@ def driver(name):
     return p
 
 
-def run(name, kernel, method, arr, iters, out0=None, timeout=300):
+def run(name, kernel, method, arr, iters, out0=None, timeout=300, wind=None):
     """Run `drive <kernel> <method>` on a host array (h, w) or (h, w, d);
-    returns (returncode, result array or None, stderr)."""
+    `wind` = (wind_x, wind_y) Args2D host arrays for the CLOUDSIM kernels.
+    Returns (returncode, result array or None, stderr)."""
     arr = np.ascontiguousarray(arr)
     out0 = arr if out0 is None else np.ascontiguousarray(out0)
     dims = arr.shape
@@ -29,7 +30,12 @@ def run(name, kernel, method, arr, iters, out0=None, timeout=300):
         fi, fo, fr = (os.path.join(t, n) for n in ("in.bin", "out0.bin", "res.bin"))
         arr.tofile(fi)
         out0.tofile(fo)
-        p = subprocess.run([driver(name), kernel, method, str(W), str(H), str(D), str(iters), fi, fo, fr],
+        extra = []
+        if wind is not None:
+            extra = [os.path.join(t, "wx.bin"), os.path.join(t, "wy.bin")]
+            np.ascontiguousarray(wind[0], np.float32).tofile(extra[0])
+            np.ascontiguousarray(wind[1], np.float32).tofile(extra[1])
+        p = subprocess.run([driver(name), kernel, method, str(W), str(H), str(D), str(iters), fi, fo, fr] + extra,
                            capture_output=True, text=True, timeout=timeout)
         res = None
         if p.returncode == 0:

@@ -50,3 +50,16 @@ def test_dropin_gxx_gpu_build_reports_missing_device_or_descriptor():
     a = inputs.jacobi2d_field(16, 16, seed=8)
     rc, _, err = dropin.run("drive_gpu_gxx", "user2d", "gpu", a, 1)
     assert rc != 0 and "GPUassert" in err
+
+
+@pytest.mark.parametrize("kernel", ["cloudsim", "usercloud"])
+@pytest.mark.parametrize("method", ["cpu", "seq"])
+def test_dropin_cpu_cloudsim_fields(kernel, method):
+    """CloudsimArgs (stock kernel) and a user Args type holding Args2D fields:
+    the CPU methods read the fields through Args2D's host map."""
+    t, wx, wy = inputs.cloudsim_fields(23, 41, seed=8)
+    for iters in (0, 1, 5):
+        rc, got, err = dropin.run("drive_cpu", kernel, method, t, iters, wind=(wx, wy))
+        assert rc == 0, err
+        want = oracle.run_2d_cloud(t, wx, wy, 0.01, iters)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (kernel, method, iters)

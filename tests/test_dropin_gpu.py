@@ -59,3 +59,27 @@ def test_dropin_gxx_rejects_user_kernel(gpu):
     a = inputs.jacobi2d_field(64, 64, seed=13)
     rc, _, err = dropin.run("drive_gpu_gxx", "user2d", "gpu", a, 3)
     assert rc == 10 + 5 and "unsupported" in err
+
+
+@pytest.mark.parametrize("binary", ["drive_gpu_gxx", "drive_gpu_hip"])
+@pytest.mark.parametrize("method", ["gpu", "pyramid", "tiling", "auto"])
+def test_dropin_cloudsim_fields_on_device(gpu, binary, method):
+    """CloudsimArgs: its wind fields (Args2D) are uploaded to HBM by the run
+    method and attached to the grid; the library's CLOUDSIM kernels read them."""
+    t, wx, wy = inputs.cloudsim_fields(150, 333, seed=9)
+    for iters in (1, 6):
+        rc, got, err = dropin.run(binary, "cloudsim", method, t, iters, wind=(wx, wy))
+        assert rc == 0, err
+        want = oracle.run_2d_cloud(t, wx, wy, 0.01, iters)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (binary, method, iters)
+
+
+def test_dropin_user_kernel_reads_fields_on_device(gpu):
+    """A user Args type with Args2D members, compiled by hipcc: the kernel reads
+    the fields' device copies (args.f(h, w) -> HBM)."""
+    t, wx, wy = inputs.cloudsim_fields(130, 270, seed=10)
+    for iters in (1, 5):
+        rc, got, err = dropin.run("drive_gpu_hip", "usercloud", "gpu", t, iters, wind=(wx, wy))
+        assert rc == 0, err
+        want = oracle.run_2d_cloud(t, wx, wy, 0.01, iters)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), iters

@@ -223,3 +223,79 @@ def test_life_padded_board_equals_dead_boundary():
     b = inputs.life_field(60, seed=3)
     got = oracle.run_2d(K.life(), b, 9)
     assert got[0].sum() == got[-1].sum() == got[:, 0].sum() == got[:, -1].sum() == 0
+
+
+# ---------------------------------------------------------------------------
+# CLOUDSIM (wind advection + conduction with field arguments)
+# ---------------------------------------------------------------------------
+def _winds(H, W, vx=0.0, vy=0.0):
+    """Args2D host arrays (shape (W, H), [y, x] with x = h, y = w) of constant wind."""
+    return np.full((W, H), vx, np.float32), np.full((W, H), vy, np.float32)
+
+
+def test_cloud_constant_field_is_fixed_point():
+    H, W = 9, 11
+    u = np.full((H, W), 3.25, np.float32)
+    for vx, vy in ((0.0, 0.0), (0.7, -0.3), (-1.0, 1.0)):
+        wx, wy = _winds(H, W, vx, vy)
+        assert np.array_equal(oracle.run_2d_cloud(u, wx, wy, 0.3, 4), u)
+
+
+@pytest.mark.parametrize("p,q", [(1, 1), (2, 3)])
+def test_cloud_conduction_eigenmode_decay(p, q):
+    """Zero wind: out = c - 0.0243 * 0.25 * dt * (4c - sum of neighbours), so the
+    zero-ring mode sin(p pi h/(H-1)) sin(q pi w/(W-1)) decays by exactly
+    1 - 0.006075 dt (4 - 2 cos a - 2 cos b) per sweep."""
+    H, W, dt, n = 33, 41, 4.0, 6
+    h = np.arange(H)[:, None]
+    w = np.arange(W)[None, :]
+    mode = np.sin(p * math.pi * h / (H - 1)) * np.sin(q * math.pi * w / (W - 1))
+    lam = 1 - 0.0243 * 0.25 * dt * (4 - 2 * math.cos(p * math.pi / (H - 1)) - 2 * math.cos(q * math.pi / (W - 1)))
+    wx, wy = _winds(H, W)
+    got = oracle.run_2d_cloud(mode.astype(np.float32), wx, wy, dt, n).astype(np.float64)
+    want = mode * lam ** n
+    assert np.max(np.abs(got - want)) < 1e-5
+
+
+@pytest.mark.parametrize("vx,vy,axis,sign", [(0.5, 0.0, 1, 1), (-0.5, 0.0, 1, -1), (0.0, 0.5, 0, 1),
+                                             (0.0, -0.5, 0, -1)])
+def test_cloud_advection_of_a_ramp(vx, vy, axis, sign):
+    """A linear ramp (slope 0.25 along w or h) has zero conduction; the wind
+    term compares each cell with its upwind-side neighbour (w + xf or h + yf,
+    xf = sign of the wind), so one sweep adds exactly 10 * 0.25 * 0.5 * dt with
+    the sign of the wind.  A wrong side or sign changes the increment."""
+    H, W, dt = 7, 9, 0.125
+    hh, ww = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    u = (1.0 + 0.25 * (ww if axis == 1 else hh)).astype(np.float32)
+    wx, wy = _winds(H, W, vx, vy)
+    got = oracle.run_2d_cloud(u, wx, wy, dt, 1)
+    want = u.copy()
+    want[1:-1, 1:-1] += np.float32(sign * 10 * 0.25 * 0.5 * dt)
+    assert np.array_equal(got, want)
+
+
+def test_cloud_field_index_map():
+    """The kernel reads wind(x = h, y = w) through Args2D's (x, y) -> y*width + x
+    (include/PSkelArgs.hpp:116-123): on a non-square grid a wind set only at
+    host element [y=w0, x=h0] moves only cell (h0, w0)."""
+    H, W, dt = 6, 10, 0.125
+    hh, ww = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    u = (1.0 + 0.25 * ww).astype(np.float32)
+    wx, wy = _winds(H, W)
+    h0, w0 = 2, 7
+    wx[w0, h0] = 0.5
+    got = oracle.run_2d_cloud(u, wx, wy, dt, 1)
+    diff = np.argwhere(got != u)
+    assert diff.tolist() == [[h0, w0]]
+    assert got[h0, w0] == u[h0, w0] + np.float32(0.15625)
+
+
+def test_cloud_ring_untouched_and_parity():
+    t, wx, wy = inputs.cloudsim_fields(12, 17, seed=3)
+    r1 = oracle.run_2d_cloud(t, wx, wy, 0.01, 1)
+    r2 = oracle.run_2d_cloud(r1, wx, wy, 0.01, 1)
+    assert np.array_equal(oracle.run_2d_cloud(t, wx, wy, 0.01, 2), r2)
+    for r in (r1, r2):
+        assert np.array_equal(r[0], t[0]) and np.array_equal(r[-1], t[-1])
+        assert np.array_equal(r[:, 0], t[:, 0]) and np.array_equal(r[:, -1], t[:, -1])
+    assert np.array_equal(oracle.run_2d_cloud(t, wx, wy, 0.01, 0), t)
