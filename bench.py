@@ -147,6 +147,28 @@ def measure(name, variant, steps, warmup, field=None, run_kw=None):
     }
 
 
+def measure_host_tiled(sweeps=64, tile=64, inner=8):
+    """Jacobi-3D 512^3 f32 out of core: the grid stays in host memory, slabs of
+    `tile` h-planes plus an `inner`-plane halo visit the device for `inner`
+    sweeps at a time (host copies included in the time)."""
+    from pskel_amd import api
+
+    f = field_of("jacobi3d")
+    s = stencil_of("jacobi3d")
+    api.run_host_tiled(s, f, inner, tile, inner)  # warm-up
+    t0 = time.perf_counter()
+    _, st = api.run_host_tiled(s, f, sweeps, tile, inner)
+    wall = time.perf_counter() - t0
+    cu = f.size * sweeps
+    return {
+        "workload": f"jacobi3d-7pt 512^3 f32 host-staged, slabs of {tile} planes, {inner} sweeps per pass, "
+                    f"{sweeps} sweeps",
+        "cell_updates_per_s": cu / wall,
+        "ms_per_sweep": wall * 1e3 / sweeps,
+        "launches": st["launches"],
+    }
+
+
 def batch_fields(seed=5):
     """The 256-grid workload: sides log-uniform in [64, 2048] per axis, half
     Jacobi f32 (interior uniform [0,1), zero ring), half life u8 (p = 0.3)."""
@@ -410,6 +432,10 @@ def main(argv=None):
                                                        "avg_launch_ms", "hbm_roofline_frac", "launch")}
             print(f"# {nm} {var}: {r['cell_updates_per_s']:.4g} CU/s {r['hbm_roofline_frac']*100:.1f}%",
                   file=sys.stderr, flush=True)
+        rh = measure_host_tiled()
+        extra["jacobi3d:host_staged"] = rh
+        print(f"# jacobi3d host-staged: {rh['cell_updates_per_s']:.4g} CU/s, {rh['ms_per_sweep']:.2f} ms/sweep",
+              file=sys.stderr, flush=True)
         rb = measure_batch(max(1, min(args.steps, 3)), 1)
         extra["batch256:graph"] = rb
         print(f"# batch256: {rb['cell_updates_per_s']:.4g} CU/s, {rb['ms_per_step']:.2f} ms/step, "

@@ -107,6 +107,7 @@ struct GpuState {
     size_t residualEvery = 0;
     std::vector<double> residuals;
     bool haveCfg = false;
+    bool hostTiling = false;
     pskel_launch_cfg_t cfg{};
     double lastMs = 0.0;
     ~GpuState()
@@ -238,14 +239,23 @@ public:
     // tiles: innerIterations is the temporal depth (sweeps per launch, at most
     // 4; 2D only), the tiling sizes are the library's.  Results are identical
     // to runIterativeGPU.
+    // Tiled run (:1663-1760).  A grid that fits in device memory stays resident
+    // and `innerIterations` becomes the temporal-tile depth (same result).  A
+    // grid that does not fit (or any grid after setHostTiling(true)) runs the
+    // reference's out-of-core scheme: slabs of `tilingHeight` rows (2D) or
+    // h-planes (3D) plus a range*inner halo visit the device, inner sweeps at a
+    // time, and the grid stays in host memory.
     void runIterativeTilingGPU(size_t iterations, size_t tilingWidth, size_t tilingHeight, size_t tilingDepth,
                                size_t innerIterations = 1, size_t GPUBlockSizeX = 0, size_t GPUBlockSizeY = 0)
     {
         (void)tilingWidth;
-        (void)tilingHeight;
         (void)tilingDepth;
         (void)GPUBlockSizeX;
         (void)GPUBlockSizeY;
+        if (gpu->hostTiling || !fitsDevice()) {
+            hostTiledRun(iterations, std::max<size_t>(1, tilingHeight), std::max<size_t>(1, innerIterations));
+            return;
+        }
         pskel_launch_cfg_t cfg{};
         cfg.variant = PSKEL_VARIANT_TUNED;
         cfg.time_tile = dims() == 2 ? (int)std::max<size_t>(1, std::min<size_t>(innerIterations, 4)) : 1;
@@ -264,6 +274,17 @@ public:
     void runIterativeAutoGPU(size_t iterations, size_t GPUBlockSize = 0)
     {
         (void)GPUBlockSize;
+        if (gpu->hostTiling || !fitsDevice()) {
+            // out of core (:2014-2074): the largest slab whose device grids fit
+            // in half the free memory, 8 sweeps per pass
+            const size_t inner = 8, halo = mask.getRange() * inner;
+            size_t freeb = 0, totalb = 0;
+            gpuErrchk(pskel_device_info(gpu->device, nullptr, 0, &freeb, &totalb, nullptr));
+            const size_t plane = input.getWidth() * (dims() == 3 ? input.getDepth() : 1) * sizeof(T);
+            const size_t slab = freeb / 2 / (3 * std::max<size_t>(plane, 1));
+            hostTiledRun(iterations, slab > 2 * halo + 1 ? slab - 2 * halo : 1, inner);
+            return;
+        }
         pskel_launch_cfg_t cfg{};
         cfg.variant = PSKEL_VARIANT_TUNED;
         gpuRun(iterations, cfg, false, /*autotune=*/true);
@@ -283,6 +304,9 @@ public:
 
     // ---- MI355X extensions ------------------------------------------------
     void setDevice(int device) { gpu->device = device; }
+    // Run runIterativeTilingGPU / runIterativeAutoGPU out of core (host-staged
+    // slabs) even when the grid fits in device memory.
+    void setHostTiling(bool on) { gpu->hostTiling = on; }
     // Device storage of the grid: PSKEL_DTYPE_BF16 / F16 keep (u - offset) in
     // 16 bits with f32 arithmetic (3D); 0 = the host element type.  `rounding`
     // is PSKEL_ROUND_NEAREST or PSKEL_ROUND_STOCHASTIC (reproducible from `seed`).
@@ -339,6 +363,47 @@ protected:
             s.gridRounding = s.rounding;
             s.gridSeed = s.seed;
             s.gridOffset = s.storeOffset;
+        }
+    }
+
+    // Would the resident grid (two buffers plus a spare) fit in free device memory?
+    bool fitsDevice() const
+    {
+        size_t freeb = 0, totalb = 0;
+        gpuErrchk(pskel_device_info(gpu->device, nullptr, 0, &freeb, &totalb, nullptr));
+        const size_t bytes = input.size() * sizeof(T);
+        return 3 * bytes + (size_t(64) << 20) < freeb;
+    }
+
+    // Out-of-core sweeps through the library (descriptor kernels only).
+    void hostTiledRun(size_t iterations, size_t tile, size_t inner)
+    {
+        if constexpr (Array::kDims == 1 || !StencilTraits<Args>::has_descriptor) {
+            gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);  // user kernels stay resident
+        } else {
+            detail::GpuState& s = *gpu;
+            const int hdt = detail::hostDtype<T>();
+            if (hdt == 0 || (s.storage != 0 && s.storage != hdt)) gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);
+            pskel_grid_desc_t gd{};
+            gd.ndim = dims();
+            gd.storage = hdt;
+            gd.width = input.getWidth();
+            gd.height = input.getHeight();
+            gd.depth = dims() == 3 ? input.getDepth() : 1;
+            gd.device = s.device;
+            pskel_stencil_desc_t desc = StencilTraits<Args>::describe(mask, args, dims());
+            std::vector<T> in(input.size()), out(output.size());
+            input.packTo(in.data());
+            output.packTo(out.data());
+            pskel_launch_cfg_t cfg{};
+            cfg.variant = PSKEL_VARIANT_TUNED;
+            if (s.haveCfg) cfg = s.cfg;
+            cfg.residual_every = 0;
+            pskel_run_stats_t st{};
+            gpuErrchk(pskel_run_iterative_host(&gd, in.data(), out.data(), hdt, &desc, iterations, tile, inner, &cfg,
+                                               &st));
+            s.lastMs = st.elapsed_ms;
+            output.unpackFrom(out.data());
         }
     }
 

@@ -208,6 +208,21 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* grid, const pskel_stencil_desc_
                                    size_t iterations, const pskel_launch_cfg_t* cfg,
                                    pskel_run_stats_t* stats);
 
+/* ---- host-staged sweeps for grids that stay in host memory:
+ *      runIterativeTilingGPU (include/PSkelStencil.hpp:1663-1760) and the tiled
+ *      branch of runIterativeAutoGPU (:2014-2074).  Passes of `inner` sweeps;
+ *      each stages slabs of `tile` rows (2D) or h-planes (3D) plus a halo of
+ *      range*inner on the device and copies the slab's core back.  `host_in` is
+ *      read only; `host_out` holds the initial output (its ring) and receives
+ *      the result, equal to `iterations` resident sweeps bit for bit.  Host
+ *      arrays are in the storage type (raw words for BF16/F16); `desc` gives
+ *      the full grid (`desc->height` rows or planes).  No residuals, no field
+ *      stencils. */
+pskel_status_t pskel_run_iterative_host(const pskel_grid_desc_t* desc, const void* host_in, void* host_out,
+                                        int32_t host_dtype, const pskel_stencil_desc_t* stencil,
+                                        size_t iterations, size_t tile, size_t inner,
+                                        const pskel_launch_cfg_t* cfg, pskel_run_stats_t* stats);
+
 /* Record one sweep launched outside the library on the grid's buffers (the
  * drop-in headers' device path for user stencil kernels with no descriptor):
  * the previous buffer becomes the current state, as runIterativeGPU's

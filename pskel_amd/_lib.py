@@ -185,6 +185,8 @@ def _declare(lib):
     _sig(lib, "pskel_grid_residual", st, P, C.POINTER(D))
     _sig(lib, "pskel_grid_swap", st, P)
     _sig(lib, "pskel_grid_set_field", st, P, I32, P)
+    _sig(lib, "pskel_run_iterative_host", st, C.POINTER(GridDesc), P, P, I32, C.POINTER(StencilDesc), SZ, SZ, SZ,
+         C.POINTER(LaunchCfg), C.POINTER(RunStats))
     _sig(lib, "pskel_field_create", st, C.POINTER(FieldDesc), C.POINTER(P))
     _sig(lib, "pskel_field_destroy", st, P)
     _sig(lib, "pskel_field_upload", st, P, P)

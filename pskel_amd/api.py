@@ -250,6 +250,39 @@ class Batch:
         self.close()
 
 
+def run_host_tiled(stencil: Stencil, inp: np.ndarray, iterations: int, tile: int, inner: int,
+                   out: Optional[np.ndarray] = None, storage: Optional[str] = None, variant: str = "tuned",
+                   launch_tile: Tuple[int, int, int] = (0, 0, 0), time_tile: int = 1, store_offset: float = 0.0,
+                   rounding: str = "nearest", seed: int = 0, device: int = 0, use_graph: bool = False):
+    """Host-staged sweeps (runIterativeTilingGPU, include/PSkelStencil.hpp:1663-1760):
+    the arrays stay in host memory, slabs of `tile` rows/planes plus a halo of
+    range*inner visit the device for `inner` sweeps at a time.  Host arrays are
+    in the storage type (raw uint16 words for bf16/f16).  Returns (result, stats)."""
+    inp = np.ascontiguousarray(inp)
+    out0 = np.ascontiguousarray(inp if out is None else out).copy()
+    if storage is None:
+        storage = {np.dtype(np.float32): "f32", np.dtype(np.uint8): "u8"}[inp.dtype]
+    st_code = _STORAGE[storage]
+    gd = L.GridDesc()
+    gd.ndim = inp.ndim
+    gd.storage = st_code
+    gd.height = inp.shape[0]
+    gd.width = inp.shape[1]
+    gd.depth = inp.shape[2] if inp.ndim == 3 else 1
+    gd.store_offset = store_offset
+    gd.device = device
+    gd.rounding = {"nearest": L.ROUND_NEAREST, "stochastic": L.ROUND_STOCHASTIC}[rounding]
+    gd.seed = int(seed) & 0xFFFFFFFF
+    cfg = make_cfg(variant, (0, 0), launch_tile, time_tile, 0, use_graph)
+    stats = L.RunStats()
+    d = stencil.desc()
+    host_dtype = {np.dtype(np.float32): L.DTYPE_F32, np.dtype(np.uint8): L.DTYPE_U8,
+                  np.dtype(np.uint16): st_code if storage in ("bf16", "f16") else 0}.get(inp.dtype, 0)
+    L.check(L.lib().pskel_run_iterative_host(C.byref(gd), inp.ctypes.data, out0.ctypes.data, host_dtype, C.byref(d),
+                                             int(iterations), int(tile), int(inner), C.byref(cfg), C.byref(stats)))
+    return out0, {"elapsed_ms": stats.elapsed_ms, "launches": stats.launches, "sweeps": stats.sweeps}
+
+
 def make_cfg(variant="plain", block=(0, 0), tile=(0, 0, 0), time_tile=1, residual_every=0,
              use_graph=False, stream=0) -> L.LaunchCfg:
     cfg = L.LaunchCfg()

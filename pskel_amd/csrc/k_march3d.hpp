@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
         float (&cur)[RW][VD] = pl[(S + 1) % 3];
         float (&nxt)[RW][VD] = pl[(S + 2) % 3];
         T* pout_h = pout + (size_t)(h - h0) * plane;
-        const uint32_t rbh = SR ? sr_plane_bits(key, h) : 0u;  // uniform: one hash per plane
+        const uint32_t rbh = SR ? sr_plane_bits(key, s.plane0 + h) : 0u;  // uniform: one hash per plane
         load_ahead(h, 2, qr[FUT], qw[FUT], qd[FUT]);
         decode(nxt, qr[NXT]);
         // stage plane h

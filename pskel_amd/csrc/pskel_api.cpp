@@ -217,6 +217,7 @@ Sweep3D make_sweep3d(const pskel_grid_t* g, const void* src, void* dst, int radi
     s.rounding = g->rounding;
     s.seed = g->seed;
     s.sweep = g->sweeps_done;
+    s.plane0 = g->plane_origin;
     s.src = src;
     s.dst = dst;
     s.W = (int)g->W;

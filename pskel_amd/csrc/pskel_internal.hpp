@@ -79,6 +79,7 @@ struct Grid {
     int rounding = PSKEL_ROUND_NEAREST;
     uint32_t seed = 0;
     uint64_t sweeps_done = 0;  // sweeps since the last INPUT upload (stochastic rounding stream)
+    uint32_t plane_origin = 0; // global h of plane 0 (a slab of a host-staged grid), for the rounding words
 
     void* buf[3] = {nullptr, nullptr, nullptr}; // [cur, other, spare]
     int parity = 0;       // which uploaded buffer's ring `cur` carries (0 = input)
@@ -121,6 +122,7 @@ struct Sweep3D {
     int rounding;       // pskel_rounding_t of 16-bit stores
     uint32_t seed;
     uint64_t sweep;     // index of the sweep this launch performs
+    uint32_t plane0;    // global h of plane 0 (stochastic rounding words use the global plane)
 };
 
 // CLOUDSIM field arguments as its kernels see them (reversed device layout:

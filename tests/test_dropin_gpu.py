@@ -31,7 +31,7 @@ def _want(stencil, a, iters):
 
 @pytest.mark.parametrize("binary", ["drive_gpu_gxx", "drive_gpu_hip"])
 @pytest.mark.parametrize("kernel,stencil,field", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("method", ["gpu", "pyramid", "tiling", "auto"])
+@pytest.mark.parametrize("method", ["gpu", "pyramid", "tiling", "auto", "hosttiling", "hostauto"])
 def test_dropin_gpu_methods(gpu, binary, kernel, stencil, field, method):
     a = field()
     for iters in (1, 6, 11):
