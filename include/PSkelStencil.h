@@ -33,10 +33,13 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <exception>
 #include <cstdint>
 #include <iostream>
 #include <memory>
 #include <type_traits>
+#include <thread>
 #include <vector>
 
 #if defined(_OPENMP)
@@ -156,6 +159,8 @@ protected:
     std::shared_ptr<detail::GpuState> gpu = std::make_shared<detail::GpuState>();
 #endif
 
+    // a stencil of the same kind, mask and args on other arrays
+    virtual std::unique_ptr<StencilBase> cloneWith(Array in, Array out) const = 0;
     virtual void runSeq(Array in, Array out, size_t width, size_t height, size_t maskRange) = 0;
     virtual void runOpenMP(Array in, Array out, size_t width, size_t height, size_t depth, size_t maskRange,
                            size_t numThreads) = 0;
@@ -292,14 +297,64 @@ public:
 
     void runAutoGPU(size_t GPUBlockSize = 0) { runIterativeAutoGPU(1, GPUBlockSize); }
 
-    // CPU+GPU split (:1288-1659): gpuFactor 1 runs on the GPU, 0 on the CPU,
-    // as the reference does (:1428-1431); a split is not implemented.
+    // CPU+GPU split (:1288-1659): the first ceil(H * gpuFactor) rows (2D) or
+    // h-planes (3D) are computed on the GPU and the rest on the CPU, at the same
+    // time, each part on a private copy of its rows plus a ghost zone of
+    // range * iterations (:1323-1325), so each part's core is exact; the cores
+    // then go to `output`.  gpuFactor 1 runs on the GPU, 0 on the CPU
+    // (:1428-1431).  `input` is left as it was.
     void runIterativePartition(size_t iterations, float gpuFactor, size_t numThreads = 0,
                                size_t GPUBlockSizeX = 32, size_t GPUBlockSizeY = 4)
     {
-        if (gpuFactor >= 1.0f) runIterativeGPU(iterations, GPUBlockSizeX, GPUBlockSizeY);
-        else if (gpuFactor <= 0.0f) runIterativeCPU(iterations, numThreads);
-        else gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);
+        const size_t H = input.getHeight();
+        const size_t gh = (size_t)std::ceil((double)H * (double)gpuFactor);
+        if (gpuFactor >= 1.0f || gh >= H) return runIterativeGPU(iterations, GPUBlockSizeX, GPUBlockSizeY);
+        if (gpuFactor <= 0.0f || gh == 0 || Array::kDims == 1) return runIterativeCPU(iterations, numThreads);
+        const size_t halo = mask.getRange() * iterations;
+        const size_t g1 = std::min(H, gh + halo), c0 = gh > halo ? gh - halo : 0;
+        const size_t W = input.getWidth(), D = input.getDepth();
+        Array sl, gin, gout, cin, cout;
+        sl.hostSlice(input, 0, 0, 0, W, g1, D);
+        gin.hostClone(sl);
+        sl.hostSlice(output, 0, 0, 0, W, g1, D);
+        gout.hostClone(sl);
+        sl.hostSlice(input, 0, c0, 0, W, H - c0, D);
+        cin.hostClone(sl);
+        sl.hostSlice(output, 0, c0, 0, W, H - c0, D);
+        cout.hostClone(sl);
+        auto gpuPart = cloneWith(gin, gout);
+        auto cpuPart = cloneWith(cin, cout);
+        detail::GpuState& mine = *gpu;
+        detail::GpuState& theirs = *gpuPart->gpu;
+        theirs.device = mine.device;
+        theirs.storage = mine.storage;
+        theirs.storeOffset = mine.storeOffset;
+        theirs.rounding = mine.rounding;
+        theirs.seed = mine.seed;
+        theirs.haveCfg = mine.haveCfg;
+        theirs.cfg = mine.cfg;
+        std::exception_ptr err;
+        std::thread gpuThread([&]() {
+            try {
+                gpuPart->runIterativeGPU(iterations, GPUBlockSizeX, GPUBlockSizeY);
+            } catch (...) {
+                err = std::current_exception();
+            }
+        });
+        cpuPart->runIterativeCPU(iterations, numThreads);
+        gpuThread.join();
+        if (err) std::rethrow_exception(err);
+        Array core;
+        sl.hostSlice(output, 0, 0, 0, W, gh, D);
+        core.hostSlice(gout, 0, 0, 0, W, gh, D);
+        sl.hostMemCopy(core);
+        sl.hostSlice(output, 0, gh, 0, W, H - gh, D);
+        core.hostSlice(cout, 0, gh - c0, 0, W, H - gh, D);
+        sl.hostMemCopy(core);
+        gin.hostFree();
+        gout.hostFree();
+        cin.hostFree();
+        cout.hostFree();
     }
 
     // ---- MI355X extensions ------------------------------------------------
@@ -551,6 +606,11 @@ protected:
                     stencilKernel(in, out, this->mask, this->args, (size_t)h, w, d);
     }
 
+    std::unique_ptr<StencilBase<Array, Mask, Args>> cloneWith(Array in, Array out) const override
+    {
+        return std::make_unique<Stencil3D>(in, out, this->mask, this->args);
+    }
+
 public:
     Stencil3D() {}
     Stencil3D(Array _input, Array _output, Mask _mask, Args _args)
@@ -590,6 +650,11 @@ protected:
             for (size_t w = maskRange; w < wrange; w++) stencilKernel(in, out, this->mask, this->args, (size_t)h, w);
     }
 
+    std::unique_ptr<StencilBase<Array, Mask, Args>> cloneWith(Array in, Array out) const override
+    {
+        return std::make_unique<Stencil2D>(in, out, this->mask, this->args);
+    }
+
 public:
     Stencil2D() {}
     Stencil2D(Array _input, Array _output, Mask _mask, Args _args)
@@ -619,6 +684,11 @@ protected:
         (void)numThreads;
 #pragma omp parallel for num_threads(numThreads) schedule(static)
         for (long long i = 0; i < (long long)width; i++) stencilKernel(in, out, this->mask, this->args, (size_t)i);
+    }
+
+    std::unique_ptr<StencilBase<Array, Mask, Args>> cloneWith(Array in, Array out) const override
+    {
+        return std::make_unique<Stencil>(in, out, this->mask, this->args);
     }
 
 public:

@@ -1,6 +1,9 @@
-"""Build libpskel_amd.so (gfx950) and the CPU oracle in-tree.
+"""Build libpskel_amd.so (gfx950) in-tree.
 
-    python -m pskel_amd.build          # rebuild everything
+    python -m pskel_amd.build          # rebuild the library
+
+(The test oracle and the drop-in drivers are built by __graft_entry__.build()
+and by the Makefiles in oracle/ and tests/cpp/.)
 """
 from __future__ import annotations
 
@@ -20,28 +23,8 @@ def build_library(force: bool = True, jobs: int = 8) -> str:
     return os.path.join(ROOT, "pskel_amd", "libpskel_amd.so")
 
 
-def build_oracle(force: bool = True) -> str:
-    cmd = ["make", "-C", os.path.join(ROOT, "oracle")]
-    if force:
-        cmd.insert(1, "-B")
-    subprocess.run(cmd, check=True)
-    return os.path.join(ROOT, "oracle", "liboracle.so")
-
-
-def build_dropin_drivers(force: bool = True) -> str:
-    """The C++ programs that exercise the drop-in headers (tests/cpp)."""
-    d = os.path.join(ROOT, "tests", "cpp")
-    cmd = ["make", "-C", d]
-    if force:
-        cmd.insert(1, "-B")
-    subprocess.run(cmd, check=True)
-    return os.path.join(d, "bin")
-
-
 def main(argv=None):
     build_library()
-    build_oracle()
-    build_dropin_drivers()
     return 0
 
 

@@ -101,6 +101,8 @@ static int runMethod(St& st, const std::string& method, size_t iters)
     else if (method == "pyramid") st.runIterativeGPU(iters, 2, 32, 4);
     else if (method == "tiling") st.runIterativeTilingGPU(iters, 256, 256, 1, 3);
     else if (method == "auto") st.runIterativeAutoGPU(iters);
+    else if (method == "partition") st.runIterativePartition(iters, 0.4f, 0, 32, 4);
+    else if (method == "partition_small") st.runIterativePartition(iters, 0.05f, 0, 32, 4);
     else if (method == "hosttiling") {
         st.setHostTiling(true);
         st.runIterativeTilingGPU(iters, 256, 7, 1, 3);

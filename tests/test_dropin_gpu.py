@@ -31,7 +31,8 @@ def _want(stencil, a, iters):
 
 @pytest.mark.parametrize("binary", ["drive_gpu_gxx", "drive_gpu_hip"])
 @pytest.mark.parametrize("kernel,stencil,field", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("method", ["gpu", "pyramid", "tiling", "auto", "hosttiling", "hostauto"])
+@pytest.mark.parametrize("method", ["gpu", "pyramid", "tiling", "auto", "hosttiling", "hostauto", "partition",
+                                    "partition_small"])
 def test_dropin_gpu_methods(gpu, binary, kernel, stencil, field, method):
     a = field()
     for iters in (1, 6, 11):
@@ -45,12 +46,14 @@ def test_dropin_gpu_methods(gpu, binary, kernel, stencil, field, method):
     ("user3d", K.jacobi3d, lambda: inputs.jacobi3d_field(36, seed=12)),
 ], ids=["user2d", "user3d"])
 def test_dropin_user_kernel_on_device(gpu, kernel, stencil, field):
-    """A user-written stencilKernel (no descriptor) compiled by hipcc for gfx950."""
+    """A user-written stencilKernel (no descriptor) compiled by hipcc for gfx950,
+    alone and split with the CPU (runIterativePartition)."""
     a = field()
     for iters in (1, 4, 9):
-        rc, got, err = dropin.run("drive_gpu_hip", kernel, "gpu", a, iters)
-        assert rc == 0, err
-        assert np.array_equal(got.view(np.uint8), _want(stencil, a, iters).view(np.uint8)), iters
+        for method in ("gpu", "partition"):
+            rc, got, err = dropin.run("drive_gpu_hip", kernel, method, a, iters)
+            assert rc == 0, err
+            assert np.array_equal(got.view(np.uint8), _want(stencil, a, iters).view(np.uint8)), (iters, method)
 
 
 def test_dropin_gxx_rejects_user_kernel(gpu):
