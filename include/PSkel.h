@@ -19,6 +19,7 @@
 #include "PSkelArray.h"
 #include "PSkelArgs.h"
 #include "PSkelMask.h"
+#include "PSkelMap.h"
 #include "PSkelStencil.h"
 #include "PSkelStencilTiling.h"
 #include "PSkelKernels.h"

@@ -69,6 +69,28 @@ __parallel__ void stencilKernel(Array2D<float> in, Array2D<float> out, Mask2D<fl
     const float result = c + -0.0243f * (sum * 0.25f) * a.dt;
     out(h, w) = result + tw * a.dt;
 }
+// A user map kernel (Map / Map2D / Map3D, include/PSkelMap.h:53-59).
+struct MapArgs {
+    float c;
+};
+
+__parallel__ void mapKernel(Array<float> in, Array<float> out, MapArgs a, size_t i)
+{
+    const float x = in(i);
+    out(i) = 0.5f * x * x + a.c;
+}
+
+__parallel__ void mapKernel(Array2D<float> in, Array2D<float> out, MapArgs a, size_t h, size_t w)
+{
+    const float x = in(h, w);
+    out(h, w) = 0.5f * x * x + a.c;
+}
+
+__parallel__ void mapKernel(Array3D<float> in, Array3D<float> out, MapArgs a, size_t h, size_t w, size_t d)
+{
+    const float x = in(h, w, d);
+    out(h, w, d) = 0.5f * x * x + a.c;
+}
 }  // namespace PSkel
 
 template <typename T>
@@ -171,6 +193,23 @@ static int runCloud(const std::string& method, size_t W, size_t H, size_t iters,
     return rc;
 }
 
+template <class M, class A>
+static int runMap(M& m, A& in, A& out, const std::string& method, size_t iters, size_t n, char** files)
+{
+    if (!readf(files[0], in.hostArray, n) || !readf(files[1], out.hostArray, n)) return 3;
+    if (method == "cpu") m.runIterativeCPU(iters, 0);
+    else if (method == "seq") m.runIterativeSequential(iters);
+#ifdef PSKEL_GPU
+    else if (method == "gpu") {
+        m.runIterativeGPU(iters);
+        if (std::getenv("PSKEL_DRIVE_TIMES")) std::fprintf(stderr, "gpu_ms %.6f\n", m.lastGpuMillis());
+    }
+#endif
+    else return 2;
+    if (!writef(files[2], out.hostArray, n)) return 4;
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc != 10 && argc != 12) {
@@ -183,6 +222,21 @@ int main(int argc, char** argv)
     char** files = argv + 7;
     try {
         if (kernel == "jacobi2d") return run2d<float>(jacobiMask2D<float>(), ScaledSumArgs{0.25, 0}, method, W, H, iters, files);
+        if (kernel == "map1d") {
+            Array<float> in(W), out(W);
+            Map<Array<float>, MapArgs> m(in, out, MapArgs{0.125f});
+            return runMap(m, in, out, method, iters, W, files);
+        }
+        if (kernel == "map2d") {
+            Array2D<float> in(W, H), out(W, H);
+            Map2D<Array2D<float>, MapArgs> m(in, out, MapArgs{0.125f});
+            return runMap(m, in, out, method, iters, W * H, files);
+        }
+        if (kernel == "map3d") {
+            Array3D<float> in(W, H, D), out(W, H, D);
+            Map3D<Array3D<float>, MapArgs> m(in, out, MapArgs{0.125f});
+            return runMap(m, in, out, method, iters, W * H * D, files);
+        }
         if (kernel == "cloudsim" || kernel == "usercloud") {
             if (argc != 12) return 2;
             if (kernel == "cloudsim") return runCloud<CloudsimArgs>(method, W, H, iters, files, argv + 10);

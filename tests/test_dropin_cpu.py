@@ -63,3 +63,21 @@ def test_dropin_cpu_cloudsim_fields(kernel, method):
         assert rc == 0, err
         want = oracle.run_2d_cloud(t, wx, wy, 0.01, iters)
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (kernel, method, iters)
+
+
+
+@pytest.mark.parametrize("kernel,shape", [("map1d", (1, 1000)), ("map2d", (37, 53)), ("map3d", (9, 11, 13))])
+@pytest.mark.parametrize("method", ["cpu", "seq"])
+def test_dropin_map_cpu(kernel, shape, method):
+    """Map / Map2D / Map3D (include/PSkelMap.h): every cell mapped, ping-pong
+    through a clone of the input, result in the output.  The user's map is
+    x -> 0.5*x*x + 0.125, applied iteratively (a plain f32 reference of it)."""
+    a = inputs.uniform(21, 0, int(np.prod(shape))).reshape(shape)
+    arr = a  # map1d: a (1, n) host array, read as Array<float>(n)
+    for iters in (0, 1, 4, 7):
+        rc, got, err = dropin.run("drive_cpu", kernel, method, arr, iters)
+        assert rc == 0, err
+        want = arr.copy()
+        for _ in range(iters):
+            want = (np.float32(0.5) * want * want + np.float32(0.125)).astype(np.float32)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (kernel, method, iters)

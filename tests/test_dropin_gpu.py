@@ -86,3 +86,21 @@ def test_dropin_user_kernel_reads_fields_on_device(gpu):
         assert rc == 0, err
         want = oracle.run_2d_cloud(t, wx, wy, 0.01, iters)
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), iters
+
+
+
+@pytest.mark.parametrize("kernel,shape", [("map1d", (1, 5000)), ("map2d", (300, 517)), ("map3d", (20, 31, 130))])
+def test_dropin_map_on_device(gpu, kernel, shape):
+    """A user mapKernel compiled by hipcc runs on the device over every cell;
+    in a g++ build the GPU method reports PSKEL_ERR_UNSUPPORTED."""
+    a = inputs.uniform(22, 0, int(np.prod(shape))).reshape(shape)
+    arr = a  # map1d: a (1, n) host array, read as Array<float>(n)
+    for iters in (0, 1, 6):
+        rc, got, err = dropin.run("drive_gpu_hip", kernel, "gpu", arr, iters)
+        assert rc == 0, err
+        want = arr.copy()
+        for _ in range(iters):
+            want = (np.float32(0.5) * want * want + np.float32(0.125)).astype(np.float32)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (kernel, iters)
+    rc, _, err = dropin.run("drive_gpu_gxx", kernel, "gpu", arr, 2)
+    assert rc == 10 + 5 and "unsupported" in err
