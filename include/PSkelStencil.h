@@ -240,10 +240,6 @@ public:
         gpuRun(iterations, cfg, false);
     }
 
-    // Tiled run (:1663-1760).  With the grid resident in HBM the tiles are LDS
-    // tiles: innerIterations is the temporal depth (sweeps per launch, at most
-    // 4; 2D only), the tiling sizes are the library's.  Results are identical
-    // to runIterativeGPU.
     // Tiled run (:1663-1760).  A grid that fits in device memory stays resident
     // and `innerIterations` becomes the temporal-tile depth (same result).  A
     // grid that does not fit (or any grid after setHostTiling(true)) runs the
