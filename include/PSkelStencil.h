@@ -276,13 +276,14 @@ public:
     {
         (void)GPUBlockSize;
         if (gpu->hostTiling || !fitsDevice()) {
-            // out of core (:2014-2074): the largest slab whose device grids fit
-            // in half the free memory, 8 sweeps per pass
+            // out of core (:2014-2074): the largest slab whose device copies
+            // (3 slots of up to 3 buffers) fit in half the free memory, 8
+            // sweeps per pass
             const size_t inner = 8, halo = mask.getRange() * inner;
             size_t freeb = 0, totalb = 0;
             gpuErrchk(pskel_device_info(gpu->device, nullptr, 0, &freeb, &totalb, nullptr));
             const size_t plane = input.getWidth() * (dims() == 3 ? input.getDepth() : 1) * sizeof(T);
-            const size_t slab = freeb / 2 / (3 * std::max<size_t>(plane, 1));
+            const size_t slab = freeb / 2 / (9 * std::max<size_t>(plane, 1));
             hostTiledRun(iterations, slab > 2 * halo + 1 ? slab - 2 * halo : 1, inner);
             return;
         }

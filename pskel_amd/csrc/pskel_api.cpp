@@ -441,6 +441,102 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
 
 }  // namespace
 
+namespace {
+
+// Validate a run and choose its kernels (shared by the resident runner and the
+// host-staged pipeline).
+pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, const pskel_launch_cfg_t& cfg,
+                        RunPlan& plan)
+{
+    const int ndim = g->ndim;
+    if (d->npoints < 1 || d->npoints > PSKEL_MAX_POINTS)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "npoints must be in [1, 64]");
+    if (d->kind != PSKEL_KIND_SCALED_SUM && d->kind != PSKEL_KIND_WEIGHTED_SUM && d->kind != PSKEL_KIND_LIFE &&
+        d->kind != PSKEL_KIND_CLOUDSIM)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown stencil kind");
+    if (d->kind == PSKEL_KIND_CLOUDSIM) {
+        if (ndim != 2 || g->storage != PSKEL_DTYPE_F32)
+            return fail(PSKEL_ERR_UNSUPPORTED, "CLOUDSIM runs on 2D f32 grids");
+        for (int k = 0; k < 2; ++k) {
+            int fd;
+            size_t fw, fh, fz, fp;
+            const float* data;
+            if (!field_info(g->fields[k], &fd, &fw, &fh, &fz, &data, &fp))
+                return fail(PSKEL_ERR_INVALID_ARGUMENT, "CLOUDSIM needs wind fields in slots 0 and 1");
+            // the kernel reads field (x = h, y = w) for every interior cell
+            if (fd != 2 || fw < g->H || fh < g->W)
+                return fail(PSKEL_ERR_INVALID_ARGUMENT, "wind fields must be 2D with width >= grid height "
+                                                        "and height >= grid width");
+        }
+        if (desc_radius(*d, 2) != 1) return fail(PSKEL_ERR_INVALID_ARGUMENT, "CLOUDSIM has mask range 1");
+    }
+    if ((d->kind == PSKEL_KIND_LIFE) != (g->storage == PSKEL_DTYPE_U8))
+        return fail(PSKEL_ERR_UNSUPPORTED, "life runs on u8 grids and u8 grids run life");
+    if (ndim == 3 && d->boundary != PSKEL_BOUNDARY_COPY && d->boundary != PSKEL_BOUNDARY_HALO)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown boundary rule");
+    const int reach = desc_reach(*d, ndim);
+    const int radius = desc_radius(*d, ndim);
+    if (radius < reach)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "mask range smaller than the farthest offset");
+    if (cfg.variant != PSKEL_VARIANT_PLAIN && cfg.variant != PSKEL_VARIANT_TUNED)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown variant");
+    if (cfg.residual_every < 0 || cfg.time_tile < 0) return fail(PSKEL_ERR_INVALID_ARGUMENT, "negative cfg field");
+    const int bx = cfg.block_x > 0 ? cfg.block_x : 64;
+    const int by = cfg.block_y > 0 ? cfg.block_y : 4;
+    if (cfg.variant == PSKEL_VARIANT_PLAIN && (bx * by > 1024 || bx > 1024 || by > 1024))
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "block has more than 1024 threads");
+    if (ndim == 3 && cfg.variant == PSKEL_VARIANT_PLAIN && cfg.residual_every > 0 && (bx * by) % 64 != 0)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "residual needs a block of whole 64-lane waves");
+
+    plan = RunPlan();
+    plan.radius = radius;
+    plan.pattern = classify(*d, ndim);
+    plan.path = 0;
+    if (cfg.variant == PSKEL_VARIANT_TUNED) {
+        const bool tunable = (ndim == 2 && (plan.pattern == PAT_J5 || plan.pattern == PAT_BOX9 ||
+                                            plan.pattern == PAT_LIFE8 || plan.pattern == PAT_CLOUD)) ||
+                             (ndim == 3 && plan.pattern == PAT_J7);
+        if (tunable) plan.path = 1;
+        // field stencils run one sweep per launch (no temporal tiles)
+        if (tunable && ndim == 2 && cfg.time_tile > 1 && plan.pattern != PAT_CLOUD) {
+            if (cfg.time_tile > 4) return fail(PSKEL_ERR_UNSUPPORTED, "temporal depth above 4");
+            if (cfg.residual_every > 0 && cfg.residual_every < cfg.time_tile)
+                return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
+            plan.path = 2;
+        }
+        if (ndim == 3 && cfg.time_tile > 1) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal tiles");
+        if (!tunable) plan.path = 0;  // no specialised kernel for this point set: the plain kernel runs
+    }
+    if (plan.path == 0) {
+        // plain launches keep the caller's block; it must have whole waves when the
+        // residual reduction uses wave shuffles
+        if (bx <= 0 || by <= 0) return fail(PSKEL_ERR_INVALID_ARGUMENT, "block dims must be positive");
+    }
+    return PSKEL_OK;
+}
+
+}  // namespace
+
+// Issue `iterations` sweeps on `stream` and return without waiting for them:
+// the host-staged pipeline overlaps them with copies on other streams.  No
+// residuals and no graph; the grid's buffers must already exist (the spare
+// too, for 2D temporal tiles of even depth).
+pskel_status_t pskel::run_async(pskel_grid_t* g, const pskel_stencil_desc_t* d, size_t iterations,
+                                const pskel_launch_cfg_t& cfg, hipStream_t stream, int64_t* launches)
+{
+    *launches = 0;
+    if (cfg.residual_every > 0 || cfg.use_graph) return fail(PSKEL_ERR_UNSUPPORTED, "residuals or graphs here");
+    RunPlan plan;
+    pskel_status_t st = plan_run(g, d, cfg, plan);
+    if (st != PSKEL_OK) return st;
+    if (iterations == 0) return PSKEL_OK;
+    if (plan.path == 2 && !g->buf[2]) return fail(PSKEL_ERR_RUNTIME, "spare buffer missing");
+    Points pts;
+    fill_points(pts, *d, g->ndim);
+    int64_t nr = 0;
+    return issue(g, *d, pts, iterations, cfg, plan, stream, launches, &nr);
+}
+
 extern "C" {
 
 int32_t pskel_abi_version(void) { return PSKEL_AMD_ABI_VERSION; }
@@ -685,70 +781,12 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
         stats->residual_count = 0;
     }
     // ---- validation -------------------------------------------------------
+    RunPlan plan;
+    if ((st = plan_run(g, d, cfg, plan)) != PSKEL_OK) return st;
     const int ndim = g->ndim;
-    if (d->npoints < 1 || d->npoints > PSKEL_MAX_POINTS)
-        return fail(PSKEL_ERR_INVALID_ARGUMENT, "npoints must be in [1, 64]");
-    if (d->kind != PSKEL_KIND_SCALED_SUM && d->kind != PSKEL_KIND_WEIGHTED_SUM && d->kind != PSKEL_KIND_LIFE &&
-        d->kind != PSKEL_KIND_CLOUDSIM)
-        return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown stencil kind");
-    if (d->kind == PSKEL_KIND_CLOUDSIM) {
-        if (ndim != 2 || g->storage != PSKEL_DTYPE_F32)
-            return fail(PSKEL_ERR_UNSUPPORTED, "CLOUDSIM runs on 2D f32 grids");
-        for (int k = 0; k < 2; ++k) {
-            int fd;
-            size_t fw, fh, fz, fp;
-            const float* data;
-            if (!field_info(g->fields[k], &fd, &fw, &fh, &fz, &data, &fp))
-                return fail(PSKEL_ERR_INVALID_ARGUMENT, "CLOUDSIM needs wind fields in slots 0 and 1");
-            // the kernel reads field (x = h, y = w) for every interior cell
-            if (fd != 2 || fw < g->H || fh < g->W)
-                return fail(PSKEL_ERR_INVALID_ARGUMENT, "wind fields must be 2D with width >= grid height "
-                                                        "and height >= grid width");
-        }
-        if (desc_radius(*d, 2) != 1) return fail(PSKEL_ERR_INVALID_ARGUMENT, "CLOUDSIM has mask range 1");
-    }
-    if ((d->kind == PSKEL_KIND_LIFE) != (g->storage == PSKEL_DTYPE_U8))
-        return fail(PSKEL_ERR_UNSUPPORTED, "life runs on u8 grids and u8 grids run life");
-    if (ndim == 3 && d->boundary != PSKEL_BOUNDARY_COPY && d->boundary != PSKEL_BOUNDARY_HALO)
-        return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown boundary rule");
-    const int reach = desc_reach(*d, ndim);
-    const int radius = desc_radius(*d, ndim);
-    if (radius < reach)
-        return fail(PSKEL_ERR_INVALID_ARGUMENT, "mask range smaller than the farthest offset");
-    if (cfg.variant != PSKEL_VARIANT_PLAIN && cfg.variant != PSKEL_VARIANT_TUNED)
-        return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown variant");
-    if (cfg.residual_every < 0 || cfg.time_tile < 0) return fail(PSKEL_ERR_INVALID_ARGUMENT, "negative cfg field");
+    const int radius = plan.radius;
     const int bx = cfg.block_x > 0 ? cfg.block_x : 64;
     const int by = cfg.block_y > 0 ? cfg.block_y : 4;
-    if (cfg.variant == PSKEL_VARIANT_PLAIN && (bx * by > 1024 || bx > 1024 || by > 1024))
-        return fail(PSKEL_ERR_INVALID_ARGUMENT, "block has more than 1024 threads");
-    if (ndim == 3 && cfg.variant == PSKEL_VARIANT_PLAIN && cfg.residual_every > 0 && (bx * by) % 64 != 0)
-        return fail(PSKEL_ERR_INVALID_ARGUMENT, "residual needs a block of whole 64-lane waves");
-
-    RunPlan plan;
-    plan.radius = radius;
-    plan.pattern = classify(*d, ndim);
-    plan.path = 0;
-    if (cfg.variant == PSKEL_VARIANT_TUNED) {
-        const bool tunable = (ndim == 2 && (plan.pattern == PAT_J5 || plan.pattern == PAT_BOX9 ||
-                                            plan.pattern == PAT_LIFE8 || plan.pattern == PAT_CLOUD)) ||
-                             (ndim == 3 && plan.pattern == PAT_J7);
-        if (tunable) plan.path = 1;
-        // field stencils run one sweep per launch (no temporal tiles)
-        if (tunable && ndim == 2 && cfg.time_tile > 1 && plan.pattern != PAT_CLOUD) {
-            if (cfg.time_tile > 4) return fail(PSKEL_ERR_UNSUPPORTED, "temporal depth above 4");
-            if (cfg.residual_every > 0 && cfg.residual_every < cfg.time_tile)
-                return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
-            plan.path = 2;
-        }
-        if (ndim == 3 && cfg.time_tile > 1) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal tiles");
-        if (!tunable) plan.path = 0;  // no specialised kernel for this point set: the plain kernel runs
-    }
-    if (plan.path == 0) {
-        // plain launches keep the caller's block; it must have whole waves when the
-        // residual reduction uses wave shuffles
-        if (bx <= 0 || by <= 0) return fail(PSKEL_ERR_INVALID_ARGUMENT, "block dims must be positive");
-    }
     if (stats) stats->sweeps = (int64_t)iterations;
     if (iterations == 0) return PSKEL_OK;
 

@@ -183,3 +183,9 @@ struct GraphEntry {
 struct pskel_grid : pskel::Grid {
     std::vector<GraphEntry> graphs;
 };
+
+namespace pskel {
+// Issue sweeps without waiting for them (the host-staged pipeline; pskel_api.cpp).
+pskel_status_t run_async(pskel_grid_t* g, const pskel_stencil_desc_t* d, size_t iterations,
+                         const pskel_launch_cfg_t& cfg, hipStream_t stream, int64_t* launches);
+}  // namespace pskel
