@@ -180,11 +180,15 @@ __device__ __forceinline__ double block_sum(double v, double* scratch /* NWAVES 
 // Map a linear block id so that consecutive logical ids land on the same XCD:
 // the dispatcher hands blocks to the 8 XCDs round-robin, so logical neighbours
 // (which share halo lines) would otherwise sit in 8 different L2s.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks)
+// XCD x receives blocks x, x+8, x+16, ...: ceil((n - x) / 8) of them, the
+// first n % 8 XCDs one more than the rest.  It gets the logical ids
+// [start(x), start(x) + count(x)), a bijection for any block count.
+__host__ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks)
 {
     constexpr uint32_t kXcd = 8;
-    if (nblocks % kXcd != 0) return bid;
-    return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
+    const uint32_t x = bid % kXcd, k = bid / kXcd;
+    const uint32_t q = nblocks / kXcd, rem = nblocks % kXcd;
+    return x * q + (x < rem ? x : rem) + k;
 }
 
 }  // namespace pskel

@@ -18,6 +18,7 @@
 // reduction order) and reduced by a second single-block pass; FP64 is used for
 // nothing else on the device.
 #include "k_march3d.hpp"
+#include "k_march3d_t2.hpp"
 
 namespace pskel {
 
@@ -162,6 +163,29 @@ hipError_t launch_3d_tuned(const Sweep3D& s, const Points& p, int storage,
     case PSKEL_DTYPE_F16: return march3d_f16(s, p, m, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+size_t ttile3d_blocks(const Sweep3D& s, const pskel_launch_cfg_t& cfg)
+{
+    const March2Shape m = march2_shape(s, cfg);
+    return (size_t)m.ntd * m.ntw * m.nseg;
+}
+
+bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg)
+{
+    if (cfg.block_x > 0 && cfg.block_x % kWave != 0) return false;
+    Sweep3D s{};
+    s.W = s.H = s.D = 1;
+    const March2Shape m = march2_shape(s, cfg);
+    return m.nwd > 0 && m.nww > 0 && m.nwd * m.nww * kWave == (cfg.block_x > 0 ? cfg.block_x : 16 * kWave) &&
+           march2_supported(m);
+}
+
+hipError_t launch_3d_ttile(const Sweep3D& s, const Points& p, int storage, const pskel_launch_cfg_t& cfg,
+                           hipStream_t st)
+{
+    if (storage != PSKEL_DTYPE_F32) return hipErrorInvalidValue;
+    return march3d_t2_f32(s, p, march2_shape(s, cfg), st);
 }
 
 hipError_t launch_reduce_partials(const double* partials, size_t n, double* out, hipStream_t st)

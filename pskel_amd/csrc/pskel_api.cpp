@@ -326,12 +326,13 @@ size_t next_k(size_t done, size_t iterations, int kmax, size_t every)
 
 }  // namespace
 
-// Ping-pong state after a launch of k fused sweeps.  Odd k wrote the other
-// buffer: swap.  Even k wrote the spare (it cannot write the buffer it reads):
-// the spare becomes current and the old current becomes the spare.
+// Ping-pong state after a launch of k fused sweeps.  Odd k, and any 3D
+// launch, wrote the other buffer: swap.  A 2D launch of even k wrote the spare
+// (the other buffer carries the other ring): the spare becomes current and
+// the old current becomes the spare.
 void pskel::advance_state(pskel::Grid& g, size_t k)
 {
-    if (k % 2) {
+    if (k % 2 || g.ndim == 3) {
         std::swap(g.buf[0], g.buf[1]);
         g.parity ^= 1;
     } else {
@@ -357,7 +358,7 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
         const bool want_res = every && ((done + k) % every == 0);
         const void* src = g->buf[0];
         void* dst;
-        const bool inplace = (k % 2) == 0;
+        const bool inplace = (k % 2) == 0 && ndim == 2;  // 3D writes every cell: no ring to keep
         if (!inplace) {
             dst = g->buf[1];
         } else {
@@ -377,6 +378,11 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
         if (plan.path == 2 && k == 1) {
             one.tile_y = 0;
             one.tile_z = 0;
+            if (ndim == 3) {
+                one.tile_x = 0;
+                one.block_x = 0;
+                one.block_y = 0;
+            }
         }
         if (ndim == 2) {
             Sweep2D s = make_sweep2d(g, src, dst, g->buf[1], plan.radius);
@@ -403,9 +409,12 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
         } else {
             Sweep3D s = make_sweep3d(g, src, dst, plan.radius, want_res ? g->d_partials : nullptr);
             size_t nb;
-            if (plan.path >= 1) {
-                nb = tuned3d_blocks(s, g->storage, cfg);
-                e = launch_3d_tuned(s, pts, g->storage, cfg, st);
+            if (k == 2) {
+                nb = ttile3d_blocks(s, cfg);
+                e = launch_3d_ttile(s, pts, g->storage, cfg, st);
+            } else if (plan.path >= 1) {
+                nb = tuned3d_blocks(s, g->storage, one);
+                e = launch_3d_tuned(s, pts, g->storage, one, st);
             } else {
                 const int bx = cfg.block_x > 0 ? cfg.block_x : 64;
                 const int by = cfg.block_y > 0 ? cfg.block_y : 4;
@@ -504,7 +513,21 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
                 return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
             plan.path = 2;
         }
-        if (ndim == 3 && cfg.time_tile > 1) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal tiles");
+        if (ndim == 3 && cfg.time_tile > 1) {
+            // two fused sweeps per launch: f32 storage, the 7-point Jacobi
+            if (cfg.time_tile > 2) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 2");
+            if (!tunable || g->storage != PSKEL_DTYPE_F32)
+                return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal tiles run the f32 7-point Jacobi");
+            if (cfg.residual_every > 0 && cfg.residual_every < cfg.time_tile)
+                return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
+            // the fused kernel addresses a plane through a buffer of < 2^31 bytes
+            if ((uint64_t)g->W * g->pitch * sizeof(float) > 0x7fffffffull)
+                return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal tiles need planes under 2 GiB");
+            if (!ttile3d_shape_ok(cfg))
+                return fail(PSKEL_ERR_UNSUPPORTED,
+                            "3D temporal tile shape (cells per lane, rows per wave, waves along d and w)");
+            plan.path = 2;
+        }
         if (!tunable) plan.path = 0;  // no specialised kernel for this point set: the plain kernel runs
     }
     if (plan.path == 0) {
@@ -530,7 +553,7 @@ pskel_status_t pskel::run_async(pskel_grid_t* g, const pskel_stencil_desc_t* d, 
     pskel_status_t st = plan_run(g, d, cfg, plan);
     if (st != PSKEL_OK) return st;
     if (iterations == 0) return PSKEL_OK;
-    if (plan.path == 2 && !g->buf[2]) return fail(PSKEL_ERR_RUNTIME, "spare buffer missing");
+    if (plan.path == 2 && g->ndim == 2 && !g->buf[2]) return fail(PSKEL_ERR_RUNTIME, "spare buffer missing");
     Points pts;
     fill_points(pts, *d, g->ndim);
     int64_t nr = 0;
@@ -801,11 +824,16 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
     if (every) {
         Sweep3D s3 = make_sweep3d(g, nullptr, nullptr, radius, nullptr);
         size_t nb = diff_blocks(g->nrows);
-        if (ndim == 3) nb = std::max(nb, plan.path >= 1 ? tuned3d_blocks(s3, g->storage, cfg) : plain3d_blocks(s3, bx, by));
+        if (ndim == 3) {
+            pskel_launch_cfg_t one = cfg;
+            if (plan.path == 2) one.tile_x = one.tile_y = one.tile_z = one.block_x = one.block_y = 0;
+            nb = std::max(nb, plan.path >= 1 ? tuned3d_blocks(s3, g->storage, one) : plain3d_blocks(s3, bx, by));
+            if (plan.path == 2) nb = std::max(nb, ttile3d_blocks(s3, cfg));
+        }
         if ((st = ensure_partials(g, nb)) != PSKEL_OK) return st;
         if ((st = ensure_res(g, nres_max)) != PSKEL_OK) return st;
     }
-    if (plan.path == 2 && !g->buf[2]) {
+    if (plan.path == 2 && ndim == 2 && !g->buf[2]) {
         PSKEL_HIP(hipMalloc(&g->buf[2], g->bytes));
         PSKEL_HIP(launch_copy(g->buf[2], g->buf[0], g->bytes, stream));
         g->spare_parity = g->parity;

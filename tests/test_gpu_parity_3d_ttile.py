@@ -1,0 +1,149 @@
+"""Two fused 3D sweeps per launch (time_tile=2, f32 7-point Jacobi) against the
+CPU oracle, bit-exact: the fused kernel evaluates both sweeps with the single
+sweep's expression, so any split of N sweeps into launches gives the same
+words.  Residuals of fused launches compare ||u^k - u^(k-1)|| against the
+oracle's to 1e-12 relative (different summation order)."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from pskel_amd import api, inputs
+from pskel_amd import kernels as K
+from pskel_amd._lib import PSkelError
+
+pytestmark = pytest.mark.gpu
+
+# (cells per lane, rows per wave, waves along d, waves along w, planes prefetched):
+# the instantiated fused-tile shapes
+SHAPES_T2 = [(2, 4, 4, 4, 1), (2, 4, 4, 2, 1), (4, 2, 2, 8, 1), (4, 2, 2, 4, 1), (2, 4, 4, 4, 2), (2, 4, 2, 4, 1),
+             (4, 4, 2, 4, 1)]
+
+# grids: tiny, one tile, ragged, many w tiles, several d tiles (D > 64 * 8)
+GRIDS = [(1, 1, 1), (2, 3, 4), (3, 3, 3), (5, 9, 70), (17, 9, 70), (33, 40, 65), (20, 70, 300), (9, 23, 1100)]
+
+
+def _field(shape, seed):
+    H, W, D = shape
+    a = inputs.uniform(seed, 0, H * W * D).reshape(shape)
+    a[0] = a[-1] = 1.0
+    a[:, 0] = a[:, -1] = 1.0
+    a[:, :, 0] = a[:, :, -1] = 1.0
+    return a
+
+
+def _block(nwd, nww, pf):
+    """cfg.block_x = threads, cfg.block_y = waves along d + 16 * (prefetch - 1)"""
+    return (64 * nwd * nww, nwd + 16 * (pf - 1))
+
+
+def _run(a, iters, tshape, hseg, **kw):
+    vd, rw, nwd, nww, pf = tshape
+    with api.Grid(a.shape, "f32") as g:
+        g.upload_pair(a)
+        st = g.run(K.jacobi3d(), iters, variant="tuned", block=_block(nwd, nww, pf), tile=(vd, rw, hseg),
+                   time_tile=2, **kw)
+        return g.download(), st
+
+
+@pytest.mark.parametrize("tshape", SHAPES_T2, ids=["vd{}_rw{}_d{}_w{}_pf{}".format(*t) for t in SHAPES_T2])
+@pytest.mark.parametrize("grid", GRIDS, ids=["x".join(map(str, g)) for g in GRIDS])
+def test_fused_pair_matches_oracle(gpu, tshape, grid):
+    a = _field(grid, sum(grid) + tshape[0])
+    for hseg in (3, 64):
+        for iters in (1, 2, 5):
+            got, st = _run(a, iters, tshape, hseg)
+            want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (tshape, grid, hseg, iters)
+            assert st["launches"] == (iters + 1) // 2
+
+
+@pytest.mark.parametrize("tshape", [(2, 4, 4, 4, 1), (4, 2, 2, 8, 1), (2, 4, 4, 4, 2)])
+def test_fused_many_tiles_and_segments(gpu, tshape):
+    """Many w tiles, several h segments of odd length, N even and odd."""
+    shape = (70, 130, 520)
+    a = _field(shape, 11)
+    for hseg, iters in ((7, 6), (32, 9), (100, 4)):
+        got, _ = _run(a, iters, tshape, hseg)
+        want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (hseg, iters)
+
+
+def test_fused_wider_range(gpu):
+    """A declared mask range of 2 (faces two cells deep) with the 7-point offsets."""
+    s = dataclasses.replace(K.jacobi3d(), radius=2)
+    shape = (21, 37, 90)
+    a = _field(shape, 12)
+    for iters in (2, 5):
+        with api.Grid(shape, "f32") as g:
+            g.upload_pair(a)
+            g.run(s, iters, variant="tuned", time_tile=2)
+            got = g.download()
+        want, _ = oracle.run_3d(s, a, iters)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), iters
+
+
+@pytest.mark.parametrize("every", [2, 4, 5])
+def test_fused_residuals(gpu, every):
+    shape = (40, 36, 100)
+    a = _field(shape, 6)
+    iters = 20
+    got_a, st = _run(a, iters, (2, 4, 4, 4, 1), 16, residual_every=every)
+    want, want_res = oracle.run_3d(K.jacobi3d(), a, iters, residual_every=every)
+    assert np.array_equal(got_a.view(np.uint32), want.view(np.uint32))
+    got = np.array(st["residuals"])
+    assert got.shape == want_res.shape == (iters // every,)
+    np.testing.assert_allclose(got, want_res, rtol=1e-12, atol=0)
+
+
+def test_fused_split_runs_and_graph(gpu):
+    """run(3); run(4) and graph replays equal run(7), and equal single sweeps."""
+    shape = (30, 50, 200)
+    a = _field(shape, 13)
+    want, _ = oracle.run_3d(K.jacobi3d(), a, 7)
+    with api.Grid(shape, "f32") as g:
+        g.upload_pair(a)
+        g.run(K.jacobi3d(), 3, variant="tuned", time_tile=2)
+        g.run(K.jacobi3d(), 4, variant="tuned", time_tile=2)
+        assert np.array_equal(g.download().view(np.uint32), want.view(np.uint32))
+    want14, _ = oracle.run_3d(K.jacobi3d(), a, 14)
+    with api.Grid(shape, "f32") as g:
+        g.upload_pair(a)
+        for rep in range(2):  # capture, then replay
+            g.run(K.jacobi3d(), 7, variant="tuned", time_tile=2, use_graph=True)
+            w = want if rep == 0 else want14
+            assert np.array_equal(g.download().view(np.uint32), w.view(np.uint32)), rep
+
+
+def test_fused_full_size(gpu):
+    """512^3 x 6 sweeps in three fused launches, bit-exact."""
+    a = inputs.jacobi3d_field(512, seed=4)
+    want, _ = oracle.run_3d(K.jacobi3d(), a, 6)
+    with api.Grid(a.shape, "f32") as g:
+        g.upload_pair(a)
+        st = g.run(K.jacobi3d(), 6, variant="tuned", time_tile=2)
+        got = g.download()
+    assert st["launches"] == 3
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_fused_rejects(gpu):
+    a = _field((10, 10, 10), 14)
+    halo = dataclasses.replace(K.jacobi3d(), boundary=K.BOUNDARY_HALO)
+    with api.Grid(a.shape, "f32") as g:
+        g.upload_pair(a)
+        with pytest.raises(PSkelError):
+            g.run(K.jacobi3d(), 4, variant="tuned", time_tile=3)
+        with pytest.raises(PSkelError):
+            g.run(halo, 4, variant="tuned", time_tile=2)
+        with pytest.raises(PSkelError):
+            g.run(K.jacobi3d(), 4, variant="tuned", tile=(8, 3, 16), time_tile=2)
+        with pytest.raises(PSkelError):  # not an instantiated shape
+            g.run(K.jacobi3d(), 4, variant="tuned", block=_block(4, 2, 2), tile=(2, 4, 16), time_tile=2)
+        with pytest.raises(PSkelError):  # threads not a whole count of waves along d
+            g.run(K.jacobi3d(), 4, variant="tuned", block=(64 * 6, 4), tile=(2, 4, 16), time_tile=2)
+    with api.Grid(a.shape, "bf16") as g:
+        g.upload_pair(a)
+        with pytest.raises(PSkelError):
+            g.run(K.jacobi3d(), 4, variant="tuned", time_tile=2)
