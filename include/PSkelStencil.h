@@ -259,7 +259,9 @@ public:
         }
         pskel_launch_cfg_t cfg{};
         cfg.variant = PSKEL_VARIANT_TUNED;
-        cfg.time_tile = dims() == 2 ? (int)std::max<size_t>(1, std::min<size_t>(innerIterations, 4)) : 1;
+        // temporal depth: up to 4 sweeps per launch in 2D, 2 in 3D (the 7-point
+        // Jacobi in f32; other 3D stencils sweep singly)
+        cfg.time_tile = (int)std::max<size_t>(1, std::min<size_t>(innerIterations, dims() == 2 ? 4 : 2));
         gpuRun(iterations, cfg, false);
     }
 

@@ -15,4 +15,6 @@ hipError_t march3d_t2_f32(const Sweep3D& s, const Points& p, const March2Shape& 
     return march2_dispatch(s, p, m, st);
 }
 
+size_t march3d_t2_f32_blocks(const Sweep3D& s, const March2Shape& m) { return march2_blocks(s, m); }
+
 }  // namespace pskel

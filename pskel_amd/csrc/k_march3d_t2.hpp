@@ -39,29 +39,32 @@
 namespace pskel {
 
 struct March2Shape {
-    int vd, rw, nwd, nww, pf, hseg, ntd, ntw, nseg;
+    int vd, rw, nwd, nww, pf;
+    int hseg;      // h planes per block; 0: chosen to fill the blocks the device holds at once
+    int ntd, ntw;  // tiles along d and w
 };
 
 inline int ceil_div_i(int a, int b) { return (a + b - 1) / b; }
 
 // cfg.tile_x = cells per lane (VD), tile_y = rows per wave (RW), tile_z = h
-// planes per block, block_x = threads per block (64 * NWD * NWW), block_y =
-// waves along d (NWD) + 16 * (planes prefetched beyond h+1, minus 1)
+// planes per block (0: chosen from the device's resident blocks), block_x =
+// threads per block (64 * NWD * NWW), block_y = waves along d (NWD) + 16 *
+// (planes prefetched beyond h+1, minus 1)
 inline March2Shape march2_shape(const Sweep3D& s, const pskel_launch_cfg_t& cfg)
 {
     March2Shape m;
-    m.vd = cfg.tile_x > 0 ? cfg.tile_x : 2;
-    m.rw = cfg.tile_y > 0 ? cfg.tile_y : 4;
-    const int nw = cfg.block_x > 0 ? cfg.block_x / kWave : 16;
-    m.nwd = cfg.block_y > 0 ? cfg.block_y % 16 : 4;
+    // default: 18-row tiles of 2 x 6 waves (16 rows advanced per tile, so a
+    // 512-wide grid takes 32 tiles and 8 segments fill the 256 CUs of an MI355X)
+    m.vd = cfg.tile_x > 0 ? cfg.tile_x : 4;
+    m.rw = cfg.tile_y > 0 ? cfg.tile_y : 3;
+    const int nw = cfg.block_x > 0 ? cfg.block_x / kWave : 12;
+    m.nwd = cfg.block_y > 0 ? cfg.block_y % 16 : 2;
     m.pf = cfg.block_y > 0 ? 1 + cfg.block_y / 16 : 1;
     m.nww = m.nwd > 0 ? nw / m.nwd : 0;
-    m.hseg = cfg.tile_z > 0 ? cfg.tile_z : 64;
-    if (m.hseg > s.H) m.hseg = s.H;
+    m.hseg = cfg.tile_z > 0 ? cfg.tile_z : 0;
     const int TD = m.nwd * kWave * m.vd, TW = m.nww * m.rw;
     m.ntd = s.D <= TD ? 1 : 1 + ceil_div_i(s.D - TD, TD - 2 * m.vd);
     m.ntw = s.W <= TW ? 1 : 1 + ceil_div_i(s.W - TW, TW - 2);
-    m.nseg = ceil_div_i(s.H, m.hseg);
     return m;
 }
 
@@ -74,10 +77,11 @@ inline bool march2_supported(const March2Shape& m)
     const int k = key(m.vd, m.rw, m.nwd, m.nww, m.pf);
     return k == key(2, 4, 4, 4, 1) || k == key(2, 4, 4, 2, 1) || k == key(4, 2, 2, 8, 1) ||
            k == key(4, 2, 2, 4, 1) || k == key(2, 4, 4, 4, 2) || k == key(2, 4, 2, 4, 1) ||
-           k == key(4, 4, 2, 4, 1);
+           k == key(4, 4, 2, 4, 1) || k == key(4, 3, 2, 6, 1);
 }
 
 hipError_t march3d_t2_f32(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st);
+size_t march3d_t2_f32_blocks(const Sweep3D& s, const March2Shape& m);  // the grid of that launch
 
 #ifdef PSKEL_MARCH_INSTANTIATE
 namespace {
@@ -146,8 +150,7 @@ __device__ __forceinline__ void lds_get(const typename LVec<VD>::T* p, float (&v
 }
 
 template <int VD, int RW, int NWD, int NWW, int PF, bool RES>
-__global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float coeff, int hseg, int ntd,
-                                                               int ntw, int nseg)
+__global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float coeff, int hseg, int ntd, int ntw)
 {
     constexpr int NW = NWD * NWW;
     constexpr int TD = NWD * kWave * VD;
@@ -165,207 +168,212 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     const int wd = wave % NWD, ww = wave / NWD;
-    const uint32_t nb = (uint32_t)(ntd * ntw * nseg);
-    const uint32_t lid = xcd_remap(blockIdx.x, nb);
-    const int td = (int)(lid % ntd);
-    const int tw = (int)((lid / ntd) % ntw);
-    const int sg = (int)(lid / ((uint32_t)ntd * ntw));
+    const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
 
     const int W = s.W, H = s.H, D = s.D, r = s.radius;
     const int pitch = (int)s.pitch;
     const size_t plane = (size_t)W * s.pitch;
+    double rr = 0.0;
 
-    // the extended tile and the cells it outputs
-    const int ed0 = td * (TD - 2 * VD);
-    const int od0 = td == 0 ? 0 : ed0 + VD;
-    const int od1 = ed0 + TD >= D ? D : ed0 + TD - VD;
-    const int ew0 = tw * (TW - 2);
-    const int ow0 = tw == 0 ? 0 : ew0 + 1;
-    const int ow1 = ew0 + TW >= W ? W : ew0 + TW - 1;
+    // one segment: planes [h0, h1) of tile (td, tw)
+    auto segment = [&](int td, int tw, int h0, int h1) __attribute__((always_inline)) {
+        // the extended tile and the cells it outputs
+        const int ed0 = td * (TD - 2 * VD);
+        const int od0 = td == 0 ? 0 : ed0 + VD;
+        const int od1 = ed0 + TD >= D ? D : ed0 + TD - VD;
+        const int ew0 = tw * (TW - 2);
+        const int ow0 = tw == 0 ? 0 : ew0 + 1;
+        const int ow1 = ew0 + TW >= W ? W : ew0 + TW - 1;
 
-    const int col = wd * kWave + lane;  // LDS word of this lane in a row
-    const int d = ed0 + col * VD;
-    const int wr = ew0 + ww * RW;
-    const int h0 = sg * hseg;
-    const int h1 = min(h0 + hseg, H);
-    const int hlo = max(h0 - 2, 0), hhi = min(h1 + 2, H);  // input planes read
-    const int ha = max(h0 - 1, 0), hb = min(h1 + 1, H);    // first-sweep planes
-    const int hend = hb == H ? hb : hb - 1;                  // a segment at the grid's end drains one more step
+        const int col = wd * kWave + lane;  // LDS word of this lane in a row
+        const int d = ed0 + col * VD;
+        const int wr = ew0 + ww * RW;
+        const int hlo = max(h0 - 2, 0), hhi = min(h1 + 2, H);  // input planes read
+        const int ha = max(h0 - 1, 0), hb = min(h1 + 1, H);    // first-sweep planes
+        const int hend = hb == H ? hb : hb - 1;                  // a segment at the grid's end drains one more step
 
-    const bool dlive = d < D;
-    const bool dout = dlive && d >= od0 && d < od1;
-    const bool dint = d >= r && d + VD - 1 < D - r;
-    bool row_int[RW], row_out[RW];
-#pragma unroll
-    for (int i = 0; i < RW; ++i) {
-        const int w = wr + i;
-        row_int[i] = w >= r && w < W - r;
-        row_out[i] = dout && w < W && w >= ow0 && w < ow1;
-    }
-    // the halo row this row of waves loads (top: the row above the tile, bottom: the row below)
-    const bool top = ww == 0, bottom = ww == NWW - 1;
-    const int hrow = top ? ew0 - 1 : ew0 + TW;
-    const int hslot = top ? 2 * NWW : 2 * NWW + 1;
-
-    // byte offsets of this lane's vectors inside a plane (plane bytes < 2^31)
-    const int plane_bytes = (int)(plane * sizeof(float));
-    uint32_t ld_off[RW], st_off[RW];
-#pragma unroll
-    for (int i = 0; i < RW; ++i) {
-        const uint32_t o = (uint32_t)(((size_t)(wr + i) * pitch + d) * sizeof(float));
-        ld_off[i] = (dlive && wr + i < W) ? o : kNoAccess;
-        st_off[i] = row_out[i] ? o : kNoAccess;
-    }
-    const uint32_t hl_off = ((top || bottom) && dlive && hrow >= 0 && hrow < W)
-                                ? (uint32_t)(((size_t)hrow * pitch + d) * sizeof(float))
-                                : kNoAccess;
-
-    constexpr int NQ = PF + 3;  // ring slots: planes h-1 .. h+PF+1
-    float in[NQ][RW][VD];       // input planes
-    float hal[NQ][VD];          // input planes of the halo row
-    float s1[NQ][RW][VD];       // first-sweep planes h-2, h-1, h (the rest unused)
-
-    auto load = [&](int p, float (&v)[RW][VD], float (&hv)[VD]) {
-        const bool ok = p >= hlo && p < hhi;  // uniform across the block
-        const auto rs = plane_rsrc((const float*)s.src + (size_t)(ok ? p : 0) * plane, ok ? plane_bytes : 0);
-#pragma unroll
-        for (int i = 0; i < RW; ++i) buf_load<VD>(rs, ld_off[i], v[i]);
-        buf_load<VD>(rs, hl_off, hv);
-    };
-    // a plane's boundary rows and edge cells into LDS
-    auto stage = [&](LV (*L)[NL], float (*E)[NWD][RW][2], const float (&v)[RW][VD]) {
-        lds_put<VD>(&L[2 * ww][col], v[0]);
-        lds_put<VD>(&L[2 * ww + 1][col], v[RW - 1]);
-        if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i < RW; ++i) E[ww][wd][i][0] = v[i][0];
-        }
-        if (lane == kWave - 1) {
-#pragma unroll
-            for (int i = 0; i < RW; ++i) E[ww][wd][i][1] = v[i][VD - 1];
-        }
-    };
-    // one sweep at plane p on this lane's cells: lo/mid/hi hold planes p-1,
-    // p, p+1 of its own rows; L/E plane p's boundary rows and edge cells.
-    // With HALO, L carries the halo rows (the input plane); without, the
-    // tile's edge rows take a stand-in neighbour (those cells are never output).
-    auto sweep = [&](auto halo, int p, const float (&lo)[RW][VD], const float (&mid)[RW][VD],
-                     const float (&hi)[RW][VD], const LV (*L)[NL], const float (*E)[NWD][RW][2],
-                     float (&out)[RW][VD]) {
-        constexpr bool HALO = decltype(halo)::value;
-        const bool pint = p >= r && p < H - r;
+        const bool dlive = d < D;
+        const bool dout = dlive && d >= od0 && d < od1;
+        const bool dint = d >= r && d + VD - 1 < D - r;
+        bool row_int[RW], row_out[RW];
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
-            float wm[VD], wp[VD];
-            if (i > 0) {
-#pragma unroll
-                for (int j = 0; j < VD; ++j) wm[j] = mid[i - 1][j];
-            } else {
-                const int slot = !top ? 2 * ww - 1 : (HALO ? 2 * NWW : 2 * ww);
-                lds_get<VD>(&L[slot][col], wm);
-            }
-            if (i < RW - 1) {
-#pragma unroll
-                for (int j = 0; j < VD; ++j) wp[j] = mid[i + 1][j];
-            } else {
-                const int slot = !bottom ? 2 * ww + 2 : (HALO ? 2 * NWW + 1 : 2 * ww + 1);
-                lds_get<VD>(&L[slot][col], wp);
-            }
-            // cells d-1 and d+VD: the neighbouring lane's, or across the wave edge
-            float left = __shfl_up(mid[i][VD - 1], 1, kWave);
-            float right = __shfl_down(mid[i][0], 1, kWave);
-            const float eleft = E[ww][wd > 0 ? wd - 1 : 0][i][1];
-            const float eright = E[ww][wd < NWD - 1 ? wd + 1 : NWD - 1][i][0];
-            if (lane == 0) left = eleft;
-            if (lane == kWave - 1) right = eright;
-            const bool face_row = !pint || !row_int[i];
-#pragma unroll
-            for (int j = 0; j < VD; ++j) {
-                const float dm = j > 0 ? mid[i][j - 1] : left;
-                const float dp = j < VD - 1 ? mid[i][j + 1] : right;
-                float acc = lo[i][j] + hi[i][j];
-                acc = acc + wm[j];
-                acc = acc + wp[j];
-                acc = acc + dm;
-                acc = acc + dp;
-                const bool face = face_row || (!dint && (d + j < r || d + j >= D - r));
-                out[i][j] = face ? mid[i][j] : coeff * acc;
-            }
+            const int w = wr + i;
+            row_int[i] = w >= r && w < W - r;
+            row_out[i] = dout && w < W && w >= ow0 && w < ow1;
         }
-    };
+        // the halo row this row of waves loads (top: the row above the tile, bottom: the row below)
+        const bool top = ww == 0, bottom = ww == NWW - 1;
+        const int hrow = top ? ew0 - 1 : ew0 + TW;
+        const int hslot = top ? 2 * NWW : 2 * NWW + 1;
+
+        // byte offsets of this lane's vectors inside a plane (plane bytes < 2^31)
+        const int plane_bytes = (int)(plane * sizeof(float));
+        uint32_t ld_off[RW], st_off[RW];
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            const uint32_t o = (uint32_t)(((size_t)(wr + i) * pitch + d) * sizeof(float));
+            ld_off[i] = (dlive && wr + i < W) ? o : kNoAccess;
+            st_off[i] = row_out[i] ? o : kNoAccess;
+        }
+        const uint32_t hl_off = ((top || bottom) && dlive && hrow >= 0 && hrow < W)
+                                    ? (uint32_t)(((size_t)hrow * pitch + d) * sizeof(float))
+                                    : kNoAccess;
+
+        constexpr int NQ = PF + 3;  // ring slots: planes h-1 .. h+PF+1
+        float in[NQ][RW][VD];       // input planes
+        float hal[NQ][VD];          // input planes of the halo row
+        float s1[NQ][RW][VD];       // first-sweep planes h-2, h-1, h (the rest unused)
+
+        auto load = [&](int p, float (&v)[RW][VD], float (&hv)[VD]) {
+            const bool ok = p >= hlo && p < hhi;  // uniform across the block
+            const auto rs = plane_rsrc((const float*)s.src + (size_t)(ok ? p : 0) * plane, ok ? plane_bytes : 0);
+#pragma unroll
+            for (int i = 0; i < RW; ++i) buf_load<VD>(rs, ld_off[i], v[i]);
+            buf_load<VD>(rs, hl_off, hv);
+        };
+        // a plane's boundary rows and edge cells into LDS
+        auto stage = [&](LV (*L)[NL], float (*E)[NWD][RW][2], const float (&v)[RW][VD]) {
+            lds_put<VD>(&L[2 * ww][col], v[0]);
+            lds_put<VD>(&L[2 * ww + 1][col], v[RW - 1]);
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < RW; ++i) E[ww][wd][i][0] = v[i][0];
+            }
+            if (lane == kWave - 1) {
+#pragma unroll
+                for (int i = 0; i < RW; ++i) E[ww][wd][i][1] = v[i][VD - 1];
+            }
+        };
+        // one sweep at plane p on this lane's cells: lo/mid/hi hold planes p-1,
+        // p, p+1 of its own rows; L/E plane p's boundary rows and edge cells.
+        // With HALO, L carries the halo rows (the input plane); without, the
+        // tile's edge rows take a stand-in neighbour (those cells are never output).
+        auto sweep = [&](auto halo, int p, const float (&lo)[RW][VD], const float (&mid)[RW][VD],
+                         const float (&hi)[RW][VD], const LV (*L)[NL], const float (*E)[NWD][RW][2],
+                         float (&out)[RW][VD]) {
+            constexpr bool HALO = decltype(halo)::value;
+            const bool pint = p >= r && p < H - r;
+#pragma unroll
+            for (int i = 0; i < RW; ++i) {
+                float wm[VD], wp[VD];
+                if (i > 0) {
+#pragma unroll
+                    for (int j = 0; j < VD; ++j) wm[j] = mid[i - 1][j];
+                } else {
+                    const int slot = !top ? 2 * ww - 1 : (HALO ? 2 * NWW : 2 * ww);
+                    lds_get<VD>(&L[slot][col], wm);
+                }
+                if (i < RW - 1) {
+#pragma unroll
+                    for (int j = 0; j < VD; ++j) wp[j] = mid[i + 1][j];
+                } else {
+                    const int slot = !bottom ? 2 * ww + 2 : (HALO ? 2 * NWW + 1 : 2 * ww + 1);
+                    lds_get<VD>(&L[slot][col], wp);
+                }
+                // cells d-1 and d+VD: the neighbouring lane's, or across the wave edge
+                float left = __shfl_up(mid[i][VD - 1], 1, kWave);
+                float right = __shfl_down(mid[i][0], 1, kWave);
+                const float eleft = E[ww][wd > 0 ? wd - 1 : 0][i][1];
+                const float eright = E[ww][wd < NWD - 1 ? wd + 1 : NWD - 1][i][0];
+                if (lane == 0) left = eleft;
+                if (lane == kWave - 1) right = eright;
+                const bool face_row = !pint || !row_int[i];
+#pragma unroll
+                for (int j = 0; j < VD; ++j) {
+                    const float dm = j > 0 ? mid[i][j - 1] : left;
+                    const float dp = j < VD - 1 ? mid[i][j + 1] : right;
+                    float acc = lo[i][j] + hi[i][j];
+                    acc = acc + wm[j];
+                    acc = acc + wp[j];
+                    acc = acc + dm;
+                    acc = acc + dp;
+                    const bool face = face_row || (!dint && (d + j < r || d + j >= D - r));
+                    out[i][j] = face ? mid[i][j] : coeff * acc;
+                }
+            }
+        };
 
 #pragma unroll
-    for (int i = 0; i < RW; ++i)
+        for (int i = 0; i < RW; ++i)
 #pragma unroll
-        for (int j = 0; j < VD; ++j) s1[0][i][j] = s1[1][i][j] = 0.f;
-    // prologue: each load followed by discarded stores, the order of the
-    // steady state, so the wait counts the compiler derives at the loop head
-    // hold on every path into it
-    {
-        const auto none = plane_rsrc(s.dst, 0);
+            for (int j = 0; j < VD; ++j) s1[0][i][j] = s1[1][i][j] = 0.f;
+        // prologue: each load followed by discarded stores, the order of the
+        // steady state, so the wait counts the compiler derives at the loop head
+        // hold on every path into it
+        {
+            const auto none = plane_rsrc(s.dst, 0);
 #pragma unroll
-        for (int k = 0; k < PF + 2; ++k) {
-            load(ha - 1 + k, in[k], hal[k]);
+            for (int k = 0; k < PF + 2; ++k) {
+                load(ha - 1 + k, in[k], hal[k]);
 #pragma unroll
-            for (int i = 0; i < RW; ++i) buf_store<VD>(none, kNoAccess, s1[0][i]);
+                for (int i = 0; i < RW; ++i) buf_store<VD>(none, kNoAccess, s1[0][i]);
+            }
         }
-    }
 
-    double rr = 0.0;
-    int b = 0;
-    // step h at ring phase S: input slot (S + k) % NQ holds plane h - 1 + k,
-    // s1 slot (S + k) % NQ the first sweep of plane h - 2 + k
-    auto step = [&](auto phase, int h) __attribute__((always_inline)) {
-        constexpr int S = decltype(phase)::value;
-        constexpr int I0 = S % NQ, I1 = (S + 1) % NQ, I2 = (S + 2) % NQ, IF = (S + NQ - 1) % NQ;
-        load(h + PF + 1, in[IF], hal[IF]);
-        const int p2 = h - 1;
-        stage(lin[b], lein[b], in[I1]);
-        if (top || bottom) lds_put<VD>(&lin[b][hslot][col], hal[I1]);
-        stage(ls1[b], les1[b], s1[I1]);
-        __syncthreads();
-        if (h < hb) {
-            sweep(std::true_type{}, h, in[I0], in[I1], in[I2], lin[b], lein[b], s1[I2]);
-        } else {
+        int b = 0;
+        // step h at ring phase S: input slot (S + k) % NQ holds plane h - 1 + k,
+        // s1 slot (S + k) % NQ the first sweep of plane h - 2 + k
+        auto step = [&](auto phase, int h) __attribute__((always_inline)) {
+            constexpr int S = decltype(phase)::value;
+            constexpr int I0 = S % NQ, I1 = (S + 1) % NQ, I2 = (S + 2) % NQ, IF = (S + NQ - 1) % NQ;
+            load(h + PF + 1, in[IF], hal[IF]);
+            const int p2 = h - 1;
+            stage(lin[b], lein[b], in[I1]);
+            if (top || bottom) lds_put<VD>(&lin[b][hslot][col], hal[I1]);
+            stage(ls1[b], les1[b], s1[I1]);
+            __syncthreads();
+            if (h < hb) {
+                sweep(std::true_type{}, h, in[I0], in[I1], in[I2], lin[b], lein[b], s1[I2]);
+            } else {
 #pragma unroll
-            for (int i = 0; i < RW; ++i)
+                for (int i = 0; i < RW; ++i)
 #pragma unroll
-                for (int j = 0; j < VD; ++j) s1[I2][i][j] = 0.f;
-        }
-        const bool out2 = p2 >= h0 && p2 < h1;  // uniform across the block
-        float o[RW][VD];
-        if (out2) {
-            sweep(std::false_type{}, p2, s1[I0], s1[I1], s1[I2], ls1[b], les1[b], o);
-            if constexpr (RES) {
+                    for (int j = 0; j < VD; ++j) s1[I2][i][j] = 0.f;
+            }
+            const bool out2 = p2 >= h0 && p2 < h1;  // uniform across the block
+            float o[RW][VD];
+            if (out2) {
+                sweep(std::false_type{}, p2, s1[I0], s1[I1], s1[I2], ls1[b], les1[b], o);
+                if constexpr (RES) {
 #pragma unroll
-                for (int i = 0; i < RW; ++i) {
-                    if (!row_out[i]) continue;
+                    for (int i = 0; i < RW; ++i) {
+                        if (!row_out[i]) continue;
 #pragma unroll
-                    for (int j = 0; j < VD; ++j) {
-                        if (d + j < D) {
-                            const double dv = (double)o[i][j] - (double)s1[I1][i][j];
-                            rr += dv * dv;
+                        for (int j = 0; j < VD; ++j) {
+                            if (d + j < D) {
+                                const double dv = (double)o[i][j] - (double)s1[I1][i][j];
+                                rr += dv * dv;
+                            }
                         }
                     }
                 }
+            } else {
+#pragma unroll
+                for (int i = 0; i < RW; ++i)
+#pragma unroll
+                    for (int j = 0; j < VD; ++j) o[i][j] = 0.f;
             }
-        } else {
+            const auto rs = plane_rsrc((float*)s.dst + (size_t)(out2 ? p2 : 0) * plane, out2 ? plane_bytes : 0);
 #pragma unroll
-            for (int i = 0; i < RW; ++i)
-#pragma unroll
-                for (int j = 0; j < VD; ++j) o[i][j] = 0.f;
+            for (int i = 0; i < RW; ++i) buf_store<VD>(rs, st_off[i], o[i]);
+            b ^= 1;
+        };
+        // whole rounds of NQ steps: a step past hend loads and stores through
+        // 0-byte resources and computes nothing that is kept
+        for (int h = ha; h <= hend; h += NQ) {
+            unroll_seq([&](auto ph) __attribute__((always_inline)) { step(ph, h + decltype(ph)::value); },
+                       std::make_integer_sequence<int, NQ>{});
         }
-        const auto rs = plane_rsrc((float*)s.dst + (size_t)(out2 ? p2 : 0) * plane, out2 ? plane_bytes : 0);
-#pragma unroll
-        for (int i = 0; i < RW; ++i) buf_store<VD>(rs, st_off[i], o[i]);
-        b ^= 1;
-    };
-    // whole rounds of NQ steps: a step past hend loads and stores through
-    // 0-byte resources and computes nothing that is kept
-    for (int h = ha; h <= hend; h += NQ) {
-        unroll_seq([&](auto ph) __attribute__((always_inline)) { step(ph, h + decltype(ph)::value); },
-                   std::make_integer_sequence<int, NQ>{});
-    }
+    };  // segment
+
+    // tiles fastest, then segments: blocks that run at once on one XCD hold
+    // neighbouring tiles at the same planes, so the rows they share are read
+    // from HBM once
+    const int tiles = ntd * ntw;
+    const int tile = (int)(lid % tiles), sg = (int)(lid / tiles);
+    const int h0 = sg * hseg;
+    segment(tile % ntd, tile / ntd, h0, min(h0 + hseg, H));
 
     if constexpr (RES) {
         const double t = block_sum<NW>(rr, red);
@@ -373,17 +381,86 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
     }
 }
 
+// Blocks and h planes per block.  cfg.tile_z fixes the planes; by default
+// the segment count n maximises the useful fraction of the device's block
+// slots (CUs x resident blocks per CU) over the rounds the grid takes, net of
+// the 4 planes a segment reads twice: (tiles * n / (rounds * slots)) *
+// (hseg / (hseg + 4)).
+template <int VD, int RW, int NWD, int NWW, int PF>
+size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
+{
+    const int64_t tiles = (int64_t)m.ntd * m.ntw;
+    int q = m.hseg > 0 ? std::min(m.hseg, s.H) : 0;
+    if (q <= 0) {
+        static int occ = 0;  // resident blocks per CU of this kernel (residual variant or not)
+        if (occ == 0) {
+            int a = 0, b = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k3d_march2<VD, RW, NWD, NWW, PF, true>,
+                                                             NWD * NWW * kWave, 0) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k3d_march2<VD, RW, NWD, NWW, PF, false>,
+                                                             NWD * NWW * kWave, 0) != hipSuccess)
+                a = b = 1;
+            occ = std::max(1, std::min(a, b));
+        }
+        static int cus_of[64] = {};  // CUs per device, queried once
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+        if (cus_of[dev] <= 0) {
+            int c = 0;
+            if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 1;
+            cus_of[dev] = c;
+        }
+        const int cus = cus_of[dev];
+        const int64_t slots = (int64_t)cus * occ;
+        double best = -1.0;
+        for (int n = 1; n <= std::min(s.H, 256); ++n) {
+            const int hs = (s.H + n - 1) / n;
+            const int64_t nb = tiles * ((s.H + hs - 1) / hs);
+            const int64_t rounds = (nb + slots - 1) / slots;
+            const double u = (double)nb / (double)(rounds * slots) * hs / (hs + 4.0);
+            if (u > best + 1e-9) {
+                best = u;
+                q = hs;
+            }
+        }
+    }
+    *hseg = q;
+    return (size_t)(tiles * ((s.H + q - 1) / q));
+}
+
 template <int VD, int RW, int NWD, int NWW, int PF>
 hipError_t march2_launch(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st)
 {
-    const dim3 grid((unsigned)m.ntd * m.ntw * m.nseg), block(NWD * NWW * kWave);
+    int hseg = 0;
+    const dim3 grid((unsigned)march2_grid<VD, RW, NWD, NWW, PF>(s, m, &hseg)), block(NWD * NWW * kWave);
     if (s.partials)
-        hipLaunchKernelGGL((k3d_march2<VD, RW, NWD, NWW, PF, true>), grid, block, 0, st, s, p.coeff, m.hseg, m.ntd,
-                           m.ntw, m.nseg);
+        hipLaunchKernelGGL((k3d_march2<VD, RW, NWD, NWW, PF, true>), grid, block, 0, st, s, p.coeff, hseg, m.ntd,
+                           m.ntw);
     else
-        hipLaunchKernelGGL((k3d_march2<VD, RW, NWD, NWW, PF, false>), grid, block, 0, st, s, p.coeff, m.hseg,
-                           m.ntd, m.ntw, m.nseg);
+        hipLaunchKernelGGL((k3d_march2<VD, RW, NWD, NWW, PF, false>), grid, block, 0, st, s, p.coeff, hseg, m.ntd,
+                           m.ntw);
     return hipGetLastError();
+}
+
+#define PSKEL_MARCH2_SHAPES(X) \
+    X(2, 4, 4, 4, 1)           \
+    X(2, 4, 4, 2, 1)           \
+    X(4, 2, 2, 8, 1)           \
+    X(4, 2, 2, 4, 1)           \
+    X(2, 4, 4, 4, 2)           \
+    X(2, 4, 2, 4, 1)           \
+    X(4, 4, 2, 4, 1)           \
+    X(4, 3, 2, 6, 1)
+
+size_t march2_blocks(const Sweep3D& s, const March2Shape& m)
+{
+    int hseg = 0;
+#define PSKEL_MARCH2(VD, RW, NWD, NWW, PF)                                       \
+    if (m.vd == VD && m.rw == RW && m.nwd == NWD && m.nww == NWW && m.pf == PF) \
+        return march2_grid<VD, RW, NWD, NWW, PF>(s, m, &hseg);
+    PSKEL_MARCH2_SHAPES(PSKEL_MARCH2)
+#undef PSKEL_MARCH2
+    return 0;
 }
 
 hipError_t march2_dispatch(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st)
@@ -392,13 +469,7 @@ hipError_t march2_dispatch(const Sweep3D& s, const Points& p, const March2Shape&
 #define PSKEL_MARCH2(VD, RW, NWD, NWW, PF)                                       \
     if (m.vd == VD && m.rw == RW && m.nwd == NWD && m.nww == NWW && m.pf == PF) \
         return march2_launch<VD, RW, NWD, NWW, PF>(s, p, m, st);
-    PSKEL_MARCH2(2, 4, 4, 4, 1)
-    PSKEL_MARCH2(2, 4, 4, 2, 1)
-    PSKEL_MARCH2(4, 2, 2, 8, 1)
-    PSKEL_MARCH2(4, 2, 2, 4, 1)
-    PSKEL_MARCH2(2, 4, 4, 4, 2)
-    PSKEL_MARCH2(2, 4, 2, 4, 1)
-    PSKEL_MARCH2(4, 4, 2, 4, 1)
+    PSKEL_MARCH2_SHAPES(PSKEL_MARCH2)
 #undef PSKEL_MARCH2
     return hipErrorInvalidValue;
 }

@@ -167,8 +167,7 @@ hipError_t launch_3d_tuned(const Sweep3D& s, const Points& p, int storage,
 
 size_t ttile3d_blocks(const Sweep3D& s, const pskel_launch_cfg_t& cfg)
 {
-    const March2Shape m = march2_shape(s, cfg);
-    return (size_t)m.ntd * m.ntw * m.nseg;
+    return march3d_t2_f32_blocks(s, march2_shape(s, cfg));
 }
 
 bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg)
@@ -177,7 +176,7 @@ bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg)
     Sweep3D s{};
     s.W = s.H = s.D = 1;
     const March2Shape m = march2_shape(s, cfg);
-    return m.nwd > 0 && m.nww > 0 && m.nwd * m.nww * kWave == (cfg.block_x > 0 ? cfg.block_x : 16 * kWave) &&
+    return m.nwd > 0 && m.nww > 0 && m.nwd * m.nww * kWave == (cfg.block_x > 0 ? cfg.block_x : 12 * kWave) &&
            march2_supported(m);
 }
 

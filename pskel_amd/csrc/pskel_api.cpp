@@ -513,11 +513,10 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
                 return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
             plan.path = 2;
         }
-        if (ndim == 3 && cfg.time_tile > 1) {
-            // two fused sweeps per launch: f32 storage, the 7-point Jacobi
-            if (cfg.time_tile > 2) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 2");
-            if (!tunable || g->storage != PSKEL_DTYPE_F32)
-                return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal tiles run the f32 7-point Jacobi");
+        if (cfg.time_tile > 2 && ndim == 3) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 2");
+        // two fused sweeps per launch run the f32 7-point Jacobi; for any
+        // other 3D stencil or storage the depth is a hint and sweeps run singly
+        if (ndim == 3 && cfg.time_tile == 2 && tunable && g->storage == PSKEL_DTYPE_F32) {
             if (cfg.residual_every > 0 && cfg.residual_every < cfg.time_tile)
                 return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
             // the fused kernel addresses a plane through a buffer of < 2^31 bytes
@@ -832,6 +831,12 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
         }
         if ((st = ensure_partials(g, nb)) != PSKEL_OK) return st;
         if ((st = ensure_res(g, nres_max)) != PSKEL_OK) return st;
+    }
+    if (plan.path == 2 && ndim == 3) {
+        // the fused kernel sizes its grid from device queries: make them here,
+        // outside any stream capture below
+        Sweep3D s3 = make_sweep3d(g, nullptr, nullptr, radius, nullptr);
+        (void)ttile3d_blocks(s3, cfg);
     }
     if (plan.path == 2 && ndim == 2 && !g->buf[2]) {
         PSKEL_HIP(hipMalloc(&g->buf[2], g->bytes));

@@ -55,6 +55,16 @@ pskel_launch_cfg_t tuned(int tx, int ty, int tz, int k)
     return c;
 }
 
+// two fused 3D sweeps per launch: tile (cells per lane, rows per wave), block
+// (threads, waves along d); the segment length is chosen at launch
+pskel_launch_cfg_t fused(int vd, int rw, int threads, int nwd)
+{
+    pskel_launch_cfg_t c = tuned(vd, rw, 0, 2);
+    c.block_x = threads;
+    c.block_y = nwd;
+    return c;
+}
+
 pskel_launch_cfg_t plain(int bx, int by)
 {
     pskel_launch_cfg_t c{};
@@ -85,6 +95,12 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
                     if (lds > 64 * 1024) continue;
                     v.push_back(tuned(vd, rw, hs, 1));
                 }
+        if (!narrow && q.iterations >= 2) {
+            v.push_back(fused(4, 3, 768, 2));
+            v.push_back(fused(4, 2, 1024, 2));
+            v.push_back(fused(4, 2, 512, 2));
+            v.push_back(fused(2, 4, 1024, 4));
+        }
         v.push_back(plain(64, 4));
     } else {
         const int blocks[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 2}, {256, 1}, {32, 4}};
@@ -140,7 +156,11 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
         return tuned(0, 2, 4, 1);
     }
     if (q.ndim == 3 && pat == PAT_J7) {
-        if (q.storage == PSKEL_DTYPE_F32) return tuned(4, 1, 128, 1);
+        if (q.storage == PSKEL_DTYPE_F32) {
+            const double cells = (double)q.width * q.height * q.depth;
+            if (q.iterations >= 2 && cells >= (1 << 21)) return fused(4, 3, 768, 2);
+            return tuned(4, 1, 128, 1);
+        }
         return q.storage == PSKEL_DTYPE_BF16 ? tuned(2, 2, 32, 1) : tuned(4, 2, 32, 1);
     }
     return plain(64, 4);

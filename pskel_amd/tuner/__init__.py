@@ -277,16 +277,21 @@ def heuristic_choice(q: Query) -> L.LaunchCfg:
 # data collection
 # ---------------------------------------------------------------------------
 def random_queries(n: int, seed: int, kinds=("jacobi2d", "diffusion9", "life", "jacobi3d")) -> List[Query]:
-    """Random shapes: 2D sides log-uniform in [64, 4096], 3D sides log-uniform
-    in [32, 512] (at most 2^27 cells); iterations log-uniform in [10, 1000]."""
+    """Random shapes: 2D sides log-uniform in [64, 4096]; 3D sides log-uniform
+    in [32, 512] for half the 3D queries and in [192, 640] for the other half,
+    where the HBM-bound kernels part ways (at most 2^27 cells); iterations
+    log-uniform in [10, 1000]."""
     rng = random.Random(seed)
     out = []
+    n3 = 0
     for i in range(n):
         kind = kinds[i % len(kinds)]
         it = int(round(math.exp(rng.uniform(math.log(10), math.log(1000)))))
         if kind == "jacobi3d":
+            lo, hi = (32, 512) if n3 % 2 == 0 else (192, 640)
+            n3 += 1
             while True:
-                s = tuple(int(round(math.exp(rng.uniform(math.log(32), math.log(512))))) for _ in range(3))
+                s = tuple(int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(3))
                 if np.prod(s) <= 2 ** 27:
                     break
             st = rng.choice(["f32", "bf16", "f16"])

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 # (cells per lane, rows per wave, waves along d, waves along w, planes prefetched):
 # the instantiated fused-tile shapes
 SHAPES_T2 = [(2, 4, 4, 4, 1), (2, 4, 4, 2, 1), (4, 2, 2, 8, 1), (4, 2, 2, 4, 1), (2, 4, 4, 4, 2), (2, 4, 2, 4, 1),
-             (4, 4, 2, 4, 1)]
+             (4, 4, 2, 4, 1), (4, 3, 2, 6, 1)]
 
 # grids: tiny, one tile, ragged, many w tiles, several d tiles (D > 64 * 8)
 GRIDS = [(1, 1, 1), (2, 3, 4), (3, 3, 3), (5, 9, 70), (17, 9, 70), (33, 40, 65), (20, 70, 300), (9, 23, 1100)]
@@ -38,11 +38,11 @@ def _block(nwd, nww, pf):
     return (64 * nwd * nww, nwd + 16 * (pf - 1))
 
 
-def _run(a, iters, tshape, hseg, **kw):
+def _run(a, iters, tshape, per, **kw):
     vd, rw, nwd, nww, pf = tshape
     with api.Grid(a.shape, "f32") as g:
         g.upload_pair(a)
-        st = g.run(K.jacobi3d(), iters, variant="tuned", block=_block(nwd, nww, pf), tile=(vd, rw, hseg),
+        st = g.run(K.jacobi3d(), iters, variant="tuned", block=_block(nwd, nww, pf), tile=(vd, rw, per),
                    time_tile=2, **kw)
         return g.download(), st
 
@@ -51,23 +51,25 @@ def _run(a, iters, tshape, hseg, **kw):
 @pytest.mark.parametrize("grid", GRIDS, ids=["x".join(map(str, g)) for g in GRIDS])
 def test_fused_pair_matches_oracle(gpu, tshape, grid):
     a = _field(grid, sum(grid) + tshape[0])
-    for hseg in (3, 64):
+    for per in (3, 0):
         for iters in (1, 2, 5):
-            got, st = _run(a, iters, tshape, hseg)
+            got, st = _run(a, iters, tshape, per)
             want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
-            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (tshape, grid, hseg, iters)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (tshape, grid, per, iters)
             assert st["launches"] == (iters + 1) // 2
 
 
-@pytest.mark.parametrize("tshape", [(2, 4, 4, 4, 1), (4, 2, 2, 8, 1), (2, 4, 4, 4, 2)])
+@pytest.mark.parametrize("tshape", [(2, 4, 4, 4, 1), (4, 2, 2, 8, 1), (2, 4, 4, 4, 2), (4, 3, 2, 6, 1)])
 def test_fused_many_tiles_and_segments(gpu, tshape):
     """Many w tiles, several h segments of odd length, N even and odd."""
     shape = (70, 130, 520)
     a = _field(shape, 11)
-    for hseg, iters in ((7, 6), (32, 9), (100, 4)):
-        got, _ = _run(a, iters, tshape, hseg)
+    # planes per block: fixed segments and the default, chosen from the
+    # device's resident blocks
+    for per, iters in ((7, 6), (32, 9), (100, 4), (0, 5)):
+        got, _ = _run(a, iters, tshape, per)
         want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
-        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (hseg, iters)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (per, iters)
 
 
 def test_fused_wider_range(gpu):
@@ -130,20 +132,35 @@ def test_fused_full_size(gpu):
 
 def test_fused_rejects(gpu):
     a = _field((10, 10, 10), 14)
-    halo = dataclasses.replace(K.jacobi3d(), boundary=K.BOUNDARY_HALO)
     with api.Grid(a.shape, "f32") as g:
         g.upload_pair(a)
         with pytest.raises(PSkelError):
             g.run(K.jacobi3d(), 4, variant="tuned", time_tile=3)
-        with pytest.raises(PSkelError):
-            g.run(halo, 4, variant="tuned", time_tile=2)
         with pytest.raises(PSkelError):
             g.run(K.jacobi3d(), 4, variant="tuned", tile=(8, 3, 16), time_tile=2)
         with pytest.raises(PSkelError):  # not an instantiated shape
             g.run(K.jacobi3d(), 4, variant="tuned", block=_block(4, 2, 2), tile=(2, 4, 16), time_tile=2)
         with pytest.raises(PSkelError):  # threads not a whole count of waves along d
             g.run(K.jacobi3d(), 4, variant="tuned", block=(64 * 6, 4), tile=(2, 4, 16), time_tile=2)
-    with api.Grid(a.shape, "bf16") as g:
+
+
+def test_fused_depth_is_a_hint(gpu):
+    """A 3D stencil or storage the fused kernel does not run takes time_tile=2
+    as a hint and sweeps singly, with the same result."""
+    a = _field((12, 13, 70), 15)
+    halo = dataclasses.replace(K.jacobi3d(), boundary=K.BOUNDARY_HALO, halo_value=0.5)
+    with api.Grid(a.shape, "f32") as g:
         g.upload_pair(a)
-        with pytest.raises(PSkelError):
-            g.run(K.jacobi3d(), 4, variant="tuned", time_tile=2)
+        st = g.run(halo, 5, variant="tuned", time_tile=2)
+        got = g.download()
+    want, _ = oracle.run_3d(halo, a, 5)
+    assert st["launches"] == 5
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    w = oracle.encode16(a, oracle.STORE_BF16)
+    with api.Grid(a.shape, "bf16") as g:
+        g.upload(0, w)
+        g.upload(1, w)
+        g.run(K.jacobi3d(), 5, variant="tuned", time_tile=2)
+        got = g.download(raw=True)
+    want16, _ = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, w, 5)
+    assert np.array_equal(got, want16)

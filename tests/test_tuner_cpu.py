@@ -35,6 +35,17 @@ def test_3d_candidates_fit_lds():
                 assert (c.tile_x in (1, 2, 4)) if st == "f32" else (c.tile_x in (2, 4, 8))
 
 
+def test_3d_fused_candidates_f32_only():
+    """Two fused sweeps per launch are offered for the f32 7-point Jacobi
+    with at least two sweeps, never for 16-bit storage."""
+    fused = lambda q: [c for c in T.candidates(q) if c.time_tile == 2]  # noqa: E731
+    assert len(fused(_q((64, 64, 64), "f32", "jacobi3d", 10))) >= 3
+    assert fused(_q((64, 64, 64), "f32", "jacobi3d", 1)) == []
+    assert fused(_q((64, 64, 64), "bf16", "jacobi3d", 10)) == []
+    h = T.heuristic_choice(_q((512, 512, 512), "f32", "jacobi3d", 100))
+    assert h.time_tile == 2
+
+
 def test_features_shape_and_determinism():
     q = _q((300, 700), "f32", "jacobi2d", 50)
     c = T.candidates(q)
