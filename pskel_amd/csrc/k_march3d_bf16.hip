@@ -1,6 +1,7 @@
 // Marching 3D kernels with bf16 storage (the template is in k_march3d.hpp).
 #define PSKEL_MARCH_INSTANTIATE
 #include "k_march3d.hpp"
+#include "k_march3d_t2.hpp"
 
 namespace pskel {
 
@@ -8,5 +9,12 @@ hipError_t march3d_bf16(const Sweep3D& s, const Points& p, const MarchShape& m, 
 {
     return march_vd<PSKEL_DTYPE_BF16>(s, p, m, st);
 }
+
+hipError_t march3d_t2_bf16(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st)
+{
+    return march2_dispatch<PSKEL_DTYPE_BF16>(s, p, m, st);
+}
+
+size_t march3d_t2_bf16_blocks(const Sweep3D& s, const March2Shape& m) { return march2_blocks<PSKEL_DTYPE_BF16>(s, m); }
 
 }  // namespace pskel

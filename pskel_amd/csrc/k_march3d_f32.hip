@@ -12,9 +12,9 @@ hipError_t march3d_f32(const Sweep3D& s, const Points& p, const MarchShape& m, h
 
 hipError_t march3d_t2_f32(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st)
 {
-    return march2_dispatch(s, p, m, st);
+    return march2_dispatch<PSKEL_DTYPE_F32>(s, p, m, st);
 }
 
-size_t march3d_t2_f32_blocks(const Sweep3D& s, const March2Shape& m) { return march2_blocks(s, m); }
+size_t march3d_t2_f32_blocks(const Sweep3D& s, const March2Shape& m) { return march2_blocks<PSKEL_DTYPE_F32>(s, m); }
 
 }  // namespace pskel

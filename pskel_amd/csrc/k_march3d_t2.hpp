@@ -1,5 +1,5 @@
 // Two fused sweeps per launch (temporal depth 2) of the 7-point scaled sum
-// with COPY faces, f32 storage.
+// with COPY faces, f32, bf16 or f16 storage.
 //
 // A block owns an extended (w, d) tile and marches along h.  At plane h it
 // computes the first sweep s1(h) on every cell of the tile from the input
@@ -25,6 +25,12 @@
 // row cross to the neighbouring wave along d.  Lane neighbours along d come
 // from lane shuffles.  LDS is double-buffered: one barrier per plane, and
 // consecutive lanes touch consecutive 8- or 16-byte words (no bank conflicts).
+//
+// 16-bit storage.  The register rings hold storage words and decode on use.
+// A first-sweep value is rounded to storage (to nearest, or stochastically
+// with the word of its own sweep index, plane and cell) before the second
+// sweep reads it, so the pair equals two stored sweeps bit for bit; faces
+// keep the input word through both sweeps.
 //
 // Global memory goes through buffer instructions whose range check drops what
 // must not happen: a plane outside the segment's reach gets a resource of 0
@@ -68,20 +74,26 @@ inline March2Shape march2_shape(const Sweep3D& s, const pskel_launch_cfg_t& cfg)
     return m;
 }
 
-// the instantiated shapes (vd, rw, nwd, nww, pf)
-inline bool march2_supported(const March2Shape& m)
+// the instantiated shapes (vd, rw, nwd, nww, pf): several for f32, two for 16-bit storage
+inline bool march2_supported(const March2Shape& m, int storage)
 {
     auto key = [](int vd, int rw, int nwd, int nww, int pf) {
         return ((((vd * 16 + rw) * 16 + nwd) * 16) + nww) * 16 + pf;
     };
     const int k = key(m.vd, m.rw, m.nwd, m.nww, m.pf);
+    if (storage != PSKEL_DTYPE_F32) return k == key(4, 3, 2, 6, 1) || k == key(4, 2, 2, 8, 1);
     return k == key(2, 4, 4, 4, 1) || k == key(2, 4, 4, 2, 1) || k == key(4, 2, 2, 8, 1) ||
            k == key(4, 2, 2, 4, 1) || k == key(2, 4, 4, 4, 2) || k == key(2, 4, 2, 4, 1) ||
            k == key(4, 4, 2, 4, 1) || k == key(4, 3, 2, 6, 1);
 }
 
 hipError_t march3d_t2_f32(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st);
-size_t march3d_t2_f32_blocks(const Sweep3D& s, const March2Shape& m);  // the grid of that launch
+hipError_t march3d_t2_bf16(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st);
+hipError_t march3d_t2_f16(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st);
+// the grid of those launches
+size_t march3d_t2_f32_blocks(const Sweep3D& s, const March2Shape& m);
+size_t march3d_t2_bf16_blocks(const Sweep3D& s, const March2Shape& m);
+size_t march3d_t2_f16_blocks(const Sweep3D& s, const March2Shape& m);
 
 #ifdef PSKEL_MARCH_INSTANTIATE
 namespace {
@@ -108,10 +120,16 @@ struct LVec<4> {
     using T = float4;
 };
 
-template <int VD>
-__device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float (&v)[VD])
+// VD consecutive elements of T (4, 8 or 16 bytes) through a buffer resource
+template <typename T, int VD>
+__device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, T (&v)[VD])
 {
-    if constexpr (VD == 2) {
+    constexpr int B = VD * (int)sizeof(T);
+    static_assert(B == 4 || B == 8 || B == 16, "lane vector of 4, 8 or 16 bytes");
+    if constexpr (B == 4) {
+        const uint32_t x = __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+        __builtin_memcpy(v, &x, 4);
+    } else if constexpr (B == 8) {
         const u32x2_t x = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
         __builtin_memcpy(v, &x, 8);
     } else {
@@ -120,10 +138,16 @@ __device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off,
     }
 }
 
-template <int VD>
-__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, const float (&v)[VD])
+template <typename T, int VD>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, const T (&v)[VD])
 {
-    if constexpr (VD == 2) {
+    constexpr int B = VD * (int)sizeof(T);
+    static_assert(B == 4 || B == 8 || B == 16, "lane vector of 4, 8 or 16 bytes");
+    if constexpr (B == 4) {
+        uint32_t x;
+        __builtin_memcpy(&x, v, 4);
+        __builtin_amdgcn_raw_buffer_store_b32(x, r, (int)off, 0, 0);
+    } else if constexpr (B == 8) {
         u32x2_t x;
         __builtin_memcpy(&x, v, 8);
         __builtin_amdgcn_raw_buffer_store_b64(x, r, (int)off, 0, 0);
@@ -149,9 +173,11 @@ __device__ __forceinline__ void lds_get(const typename LVec<VD>::T* p, float (&v
     __builtin_memcpy(v, &x, sizeof(x));
 }
 
-template <int VD, int RW, int NWD, int NWW, int PF, bool RES>
+template <int ST, int VD, int RW, int NWD, int NWW, int PF, bool RES, bool SR>
 __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float coeff, int hseg, int ntd, int ntw)
 {
+    using Tr = Stor<ST>;
+    using T = typename Tr::T;  // storage word
     constexpr int NW = NWD * NWW;
     constexpr int TD = NWD * kWave * VD;
     constexpr int TW = NWW * RW;
@@ -173,6 +199,10 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
     const int W = s.W, H = s.H, D = s.D, r = s.radius;
     const int pitch = (int)s.pitch;
     const size_t plane = (size_t)W * s.pitch;
+    const float off = s.store_offset;
+    // stochastic rounding: the keys of the launch's two sweeps
+    const uint32_t key1 = SR ? sweep_key(s.seed, s.sweep) : 0u;
+    const uint32_t key2 = SR ? sweep_key(s.seed, s.sweep + 1) : 0u;
     double rr = 0.0;
 
     // one segment: planes [h0, h1) of tile (td, tw)
@@ -208,29 +238,36 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         const int hslot = top ? 2 * NWW : 2 * NWW + 1;
 
         // byte offsets of this lane's vectors inside a plane (plane bytes < 2^31)
-        const int plane_bytes = (int)(plane * sizeof(float));
+        const int plane_bytes = (int)(plane * sizeof(T));
         uint32_t ld_off[RW], st_off[RW];
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
-            const uint32_t o = (uint32_t)(((size_t)(wr + i) * pitch + d) * sizeof(float));
+            const uint32_t o = (uint32_t)(((size_t)(wr + i) * pitch + d) * sizeof(T));
             ld_off[i] = (dlive && wr + i < W) ? o : kNoAccess;
             st_off[i] = row_out[i] ? o : kNoAccess;
         }
         const uint32_t hl_off = ((top || bottom) && dlive && hrow >= 0 && hrow < W)
-                                    ? (uint32_t)(((size_t)hrow * pitch + d) * sizeof(float))
+                                    ? (uint32_t)(((size_t)hrow * pitch + d) * sizeof(T))
                                     : kNoAccess;
 
         constexpr int NQ = PF + 3;  // ring slots: planes h-1 .. h+PF+1
-        float in[NQ][RW][VD];       // input planes
-        float hal[NQ][VD];          // input planes of the halo row
-        float s1[NQ][RW][VD];       // first-sweep planes h-2, h-1, h (the rest unused)
+        T in[NQ][RW][VD];           // input planes (storage words)
+        T hal[NQ][VD];              // input planes of the halo row
+        T s1[NQ][RW][VD];           // first-sweep planes h-2, h-1, h (the rest unused), storage words
 
-        auto load = [&](int p, float (&v)[RW][VD], float (&hv)[VD]) {
+
+        auto load = [&](int p, T (&v)[RW][VD], T (&hv)[VD]) {
             const bool ok = p >= hlo && p < hhi;  // uniform across the block
-            const auto rs = plane_rsrc((const float*)s.src + (size_t)(ok ? p : 0) * plane, ok ? plane_bytes : 0);
+            const auto rs = plane_rsrc((const T*)s.src + (size_t)(ok ? p : 0) * plane, ok ? plane_bytes : 0);
 #pragma unroll
-            for (int i = 0; i < RW; ++i) buf_load<VD>(rs, ld_off[i], v[i]);
-            buf_load<VD>(rs, hl_off, hv);
+            for (int i = 0; i < RW; ++i) buf_load<T, VD>(rs, ld_off[i], v[i]);
+            buf_load<T, VD>(rs, hl_off, hv);
+        };
+        auto decode = [&](const T (&w)[RW][VD], float (&f)[RW][VD]) {
+#pragma unroll
+            for (int i = 0; i < RW; ++i)
+#pragma unroll
+                for (int j = 0; j < VD; ++j) f[i][j] = Tr::dec(w[i][j], off);
         };
         // a plane's boundary rows and edge cells into LDS
         auto stage = [&](LV (*L)[NL], float (*E)[NWD][RW][2], const float (&v)[RW][VD]) {
@@ -249,11 +286,24 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         // p, p+1 of its own rows; L/E plane p's boundary rows and edge cells.
         // With HALO, L carries the halo rows (the input plane); without, the
         // tile's edge rows take a stand-in neighbour (those cells are never output).
+        // out: storage words; a face keeps its word `midw`, any other cell rounds
+        // coeff * sum (stochastically: with the word of its sweep `key`, plane
+        // and cell, hashed here, where issue slots are spare and registers not)
         auto sweep = [&](auto halo, int p, const float (&lo)[RW][VD], const float (&mid)[RW][VD],
-                         const float (&hi)[RW][VD], const LV (*L)[NL], const float (*E)[NWD][RW][2],
-                         float (&out)[RW][VD]) {
+                         const float (&hi)[RW][VD], const T (&midw)[RW][VD], const LV (*L)[NL],
+                         const float (*E)[NWD][RW][2], uint32_t key, T (&out)[RW][VD]) {
             constexpr bool HALO = decltype(halo)::value;
             const bool pint = p >= r && p < H - r;
+            const uint32_t rb = SR ? sr_plane_bits(key, s.plane0 + (uint32_t)p) : 0u;  // uniform
+            // The in-plane hashes do not change along h; opaque copies of the
+            // key and of this lane's first cell index keep the compiler from
+            // hoisting all RW x VD of them into registers (an empty asm, no
+            // instruction).
+            uint32_t kk = key, cell0 = (uint32_t)wr * (uint32_t)D + (uint32_t)d;
+            if constexpr (SR) {
+                asm volatile("" : "+s"(kk));
+                asm volatile("" : "+v"(cell0));
+            }
 #pragma unroll
             for (int i = 0; i < RW; ++i) {
                 float wm[VD], wp[VD];
@@ -289,7 +339,14 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
                     acc = acc + dm;
                     acc = acc + dp;
                     const bool face = face_row || (!dint && (d + j < r || d + j >= D - r));
-                    out[i][j] = face ? mid[i][j] : coeff * acc;
+                    T v;
+                    if constexpr (SR) {
+                        const uint32_t wd_idx = cell0 + (uint32_t)i * (uint32_t)D + (uint32_t)j;
+                        v = Tr::enc_sr(coeff * acc, off, sr_inplane_bits(kk, wd_idx) ^ rb);
+                    } else {
+                        v = Tr::enc(coeff * acc, off);
+                    }
+                    out[i][j] = face ? midw[i][j] : v;
                 }
             }
         };
@@ -297,7 +354,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
         for (int i = 0; i < RW; ++i)
 #pragma unroll
-            for (int j = 0; j < VD; ++j) s1[0][i][j] = s1[1][i][j] = 0.f;
+            for (int j = 0; j < VD; ++j) s1[0][i][j] = s1[1][i][j] = T(0);
         // prologue: each load followed by discarded stores, the order of the
         // steady state, so the wait counts the compiler derives at the loop head
         // hold on every path into it
@@ -307,7 +364,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
             for (int k = 0; k < PF + 2; ++k) {
                 load(ha - 1 + k, in[k], hal[k]);
 #pragma unroll
-                for (int i = 0; i < RW; ++i) buf_store<VD>(none, kNoAccess, s1[0][i]);
+                for (int i = 0; i < RW; ++i) buf_store<T, VD>(none, kNoAccess, s1[0][i]);
             }
         }
 
@@ -319,22 +376,36 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
             constexpr int I0 = S % NQ, I1 = (S + 1) % NQ, I2 = (S + 2) % NQ, IF = (S + NQ - 1) % NQ;
             load(h + PF + 1, in[IF], hal[IF]);
             const int p2 = h - 1;
-            stage(lin[b], lein[b], in[I1]);
-            if (top || bottom) lds_put<VD>(&lin[b][hslot][col], hal[I1]);
-            stage(ls1[b], les1[b], s1[I1]);
+            float fm[RW][VD], gm[RW][VD];  // input plane h and s1(h-1), decoded
+            decode(in[I1], fm);
+            decode(s1[I1], gm);
+            stage(lin[b], lein[b], fm);
+            if (top || bottom) {
+                float fh[VD];
+#pragma unroll
+                for (int j = 0; j < VD; ++j) fh[j] = Tr::dec(hal[I1][j], off);
+                lds_put<VD>(&lin[b][hslot][col], fh);
+            }
+            stage(ls1[b], les1[b], gm);
             __syncthreads();
             if (h < hb) {
-                sweep(std::true_type{}, h, in[I0], in[I1], in[I2], lin[b], lein[b], s1[I2]);
+                float flo[RW][VD], fhi[RW][VD];
+                decode(in[I0], flo);
+                decode(in[I2], fhi);
+                sweep(std::true_type{}, h, flo, fm, fhi, in[I1], lin[b], lein[b], key1, s1[I2]);
             } else {
 #pragma unroll
                 for (int i = 0; i < RW; ++i)
 #pragma unroll
-                    for (int j = 0; j < VD; ++j) s1[I2][i][j] = 0.f;
+                    for (int j = 0; j < VD; ++j) s1[I2][i][j] = T(0);
             }
             const bool out2 = p2 >= h0 && p2 < h1;  // uniform across the block
-            float o[RW][VD];
+            T o[RW][VD];
             if (out2) {
-                sweep(std::false_type{}, p2, s1[I0], s1[I1], s1[I2], ls1[b], les1[b], o);
+                float glo[RW][VD], ghi[RW][VD];
+                decode(s1[I0], glo);
+                decode(s1[I2], ghi);
+                sweep(std::false_type{}, p2, glo, gm, ghi, s1[I1], ls1[b], les1[b], key2, o);
                 if constexpr (RES) {
 #pragma unroll
                     for (int i = 0; i < RW; ++i) {
@@ -342,7 +413,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
                         for (int j = 0; j < VD; ++j) {
                             if (d + j < D) {
-                                const double dv = (double)o[i][j] - (double)s1[I1][i][j];
+                                const double dv = (double)Tr::dec_raw(o[i][j]) - (double)Tr::dec_raw(s1[I1][i][j]);
                                 rr += dv * dv;
                             }
                         }
@@ -352,11 +423,11 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
                 for (int i = 0; i < RW; ++i)
 #pragma unroll
-                    for (int j = 0; j < VD; ++j) o[i][j] = 0.f;
+                    for (int j = 0; j < VD; ++j) o[i][j] = T(0);
             }
-            const auto rs = plane_rsrc((float*)s.dst + (size_t)(out2 ? p2 : 0) * plane, out2 ? plane_bytes : 0);
+            const auto rs = plane_rsrc((T*)s.dst + (size_t)(out2 ? p2 : 0) * plane, out2 ? plane_bytes : 0);
 #pragma unroll
-            for (int i = 0; i < RW; ++i) buf_store<VD>(rs, st_off[i], o[i]);
+            for (int i = 0; i < RW; ++i) buf_store<T, VD>(rs, st_off[i], o[i]);
             b ^= 1;
         };
         // whole rounds of NQ steps: a step past hend loads and stores through
@@ -386,21 +457,28 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 // slots (CUs x resident blocks per CU) over the rounds the grid takes, net of
 // the 4 planes a segment reads twice: (tiles * n / (rounds * slots)) *
 // (hseg / (hseg + 4)).
-template <int VD, int RW, int NWD, int NWW, int PF>
+template <int ST, int VD, int RW, int NWD, int NWW, int PF>
 size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
 {
     const int64_t tiles = (int64_t)m.ntd * m.ntw;
     int q = m.hseg > 0 ? std::min(m.hseg, s.H) : 0;
     if (q <= 0) {
-        static int occ = 0;  // resident blocks per CU of this kernel (residual variant or not)
+        // resident blocks per CU of this kernel: the least over its variants
+        static int occ = 0;
         if (occ == 0) {
-            int a = 0, b = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k3d_march2<VD, RW, NWD, NWW, PF, true>,
-                                                             NWD * NWW * kWave, 0) != hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k3d_march2<VD, RW, NWD, NWW, PF, false>,
-                                                             NWD * NWW * kWave, 0) != hipSuccess)
-                a = b = 1;
-            occ = std::max(1, std::min(a, b));
+            constexpr int NT = NWD * NWW * kWave;
+            int o[4] = {1, 1, 1, 1};
+            bool ok = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &o[0], k3d_march2<ST, VD, RW, NWD, NWW, PF, true, false>, NT, 0) == hipSuccess &&
+                      hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &o[1], k3d_march2<ST, VD, RW, NWD, NWW, PF, false, false>, NT, 0) == hipSuccess;
+            if constexpr (ST != PSKEL_DTYPE_F32)
+                ok = ok &&
+                     hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &o[2], k3d_march2<ST, VD, RW, NWD, NWW, PF, true, true>, NT, 0) == hipSuccess &&
+                     hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &o[3], k3d_march2<ST, VD, RW, NWD, NWW, PF, false, true>, NT, 0) == hipSuccess;
+            occ = ok ? std::max(1, std::min(std::min(o[0], o[1]), std::min(o[2], o[3]))) : 1;
         }
         static int cus_of[64] = {};  // CUs per device, queried once
         int dev = 0;
@@ -410,8 +488,7 @@ size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
             if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 1;
             cus_of[dev] = c;
         }
-        const int cus = cus_of[dev];
-        const int64_t slots = (int64_t)cus * occ;
+        const int64_t slots = (int64_t)cus_of[dev] * occ;
         double best = -1.0;
         for (int n = 1; n <= std::min(s.H, 256); ++n) {
             const int hs = (s.H + n - 1) / n;
@@ -428,48 +505,70 @@ size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
     return (size_t)(tiles * ((s.H + q - 1) / q));
 }
 
-template <int VD, int RW, int NWD, int NWW, int PF>
+template <int ST, int VD, int RW, int NWD, int NWW, int PF>
 hipError_t march2_launch(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st)
 {
     int hseg = 0;
-    const dim3 grid((unsigned)march2_grid<VD, RW, NWD, NWW, PF>(s, m, &hseg)), block(NWD * NWW * kWave);
-    if (s.partials)
-        hipLaunchKernelGGL((k3d_march2<VD, RW, NWD, NWW, PF, true>), grid, block, 0, st, s, p.coeff, hseg, m.ntd,
-                           m.ntw);
-    else
-        hipLaunchKernelGGL((k3d_march2<VD, RW, NWD, NWW, PF, false>), grid, block, 0, st, s, p.coeff, hseg, m.ntd,
-                           m.ntw);
+    const dim3 grid((unsigned)march2_grid<ST, VD, RW, NWD, NWW, PF>(s, m, &hseg)), block(NWD * NWW * kWave);
+    const bool sr = ST != PSKEL_DTYPE_F32 && s.rounding == PSKEL_ROUND_STOCHASTIC;
+#define PSKEL_MARCH2_GO(RESV, SRV)                                                                           \
+    hipLaunchKernelGGL((k3d_march2<ST, VD, RW, NWD, NWW, PF, RESV, SRV>), grid, block, 0, st, s, p.coeff, hseg, \
+                       m.ntd, m.ntw)
+    if constexpr (ST == PSKEL_DTYPE_F32) {
+        if (s.partials) PSKEL_MARCH2_GO(true, false);
+        else PSKEL_MARCH2_GO(false, false);
+    } else {
+        if (s.partials && sr) PSKEL_MARCH2_GO(true, true);
+        else if (s.partials) PSKEL_MARCH2_GO(true, false);
+        else if (sr) PSKEL_MARCH2_GO(false, true);
+        else PSKEL_MARCH2_GO(false, false);
+    }
+#undef PSKEL_MARCH2_GO
     return hipGetLastError();
 }
 
-#define PSKEL_MARCH2_SHAPES(X) \
-    X(2, 4, 4, 4, 1)           \
-    X(2, 4, 4, 2, 1)           \
-    X(4, 2, 2, 8, 1)           \
-    X(4, 2, 2, 4, 1)           \
-    X(2, 4, 4, 4, 2)           \
-    X(2, 4, 2, 4, 1)           \
-    X(4, 4, 2, 4, 1)           \
+// instantiated shapes (vd, rw, nwd, nww, pf), as march2_supported lists them
+#define PSKEL_MARCH2_SHAPES_F32(X) \
+    X(2, 4, 4, 4, 1)               \
+    X(2, 4, 4, 2, 1)               \
+    X(4, 2, 2, 8, 1)               \
+    X(4, 2, 2, 4, 1)               \
+    X(2, 4, 4, 4, 2)               \
+    X(2, 4, 2, 4, 1)               \
+    X(4, 4, 2, 4, 1)               \
     X(4, 3, 2, 6, 1)
+#define PSKEL_MARCH2_SHAPES_16(X) \
+    X(4, 3, 2, 6, 1)              \
+    X(4, 2, 2, 8, 1)
 
+template <int ST>
 size_t march2_blocks(const Sweep3D& s, const March2Shape& m)
 {
     int hseg = 0;
 #define PSKEL_MARCH2(VD, RW, NWD, NWW, PF)                                       \
     if (m.vd == VD && m.rw == RW && m.nwd == NWD && m.nww == NWW && m.pf == PF) \
-        return march2_grid<VD, RW, NWD, NWW, PF>(s, m, &hseg);
-    PSKEL_MARCH2_SHAPES(PSKEL_MARCH2)
+        return march2_grid<ST, VD, RW, NWD, NWW, PF>(s, m, &hseg);
+    if constexpr (ST == PSKEL_DTYPE_F32) {
+        PSKEL_MARCH2_SHAPES_F32(PSKEL_MARCH2)
+    } else {
+        PSKEL_MARCH2_SHAPES_16(PSKEL_MARCH2)
+    }
 #undef PSKEL_MARCH2
     return 0;
 }
 
+template <int ST>
 hipError_t march2_dispatch(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st)
 {
-    if (!march2_supported(m)) return hipErrorInvalidValue;
+    if (!march2_supported(m, ST)) return hipErrorInvalidValue;
 #define PSKEL_MARCH2(VD, RW, NWD, NWW, PF)                                       \
     if (m.vd == VD && m.rw == RW && m.nwd == NWD && m.nww == NWW && m.pf == PF) \
-        return march2_launch<VD, RW, NWD, NWW, PF>(s, p, m, st);
-    PSKEL_MARCH2_SHAPES(PSKEL_MARCH2)
+        return march2_launch<ST, VD, RW, NWD, NWW, PF>(s, p, m, st);
+    if constexpr (ST == PSKEL_DTYPE_F32) {
+        PSKEL_MARCH2_SHAPES_F32(PSKEL_MARCH2)
+    } else {
+        PSKEL_MARCH2_SHAPES_16(PSKEL_MARCH2)
+    }
 #undef PSKEL_MARCH2
     return hipErrorInvalidValue;
 }

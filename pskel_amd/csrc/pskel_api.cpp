@@ -410,7 +410,7 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
             Sweep3D s = make_sweep3d(g, src, dst, plan.radius, want_res ? g->d_partials : nullptr);
             size_t nb;
             if (k == 2) {
-                nb = ttile3d_blocks(s, cfg);
+                nb = ttile3d_blocks(s, g->storage, cfg);
                 e = launch_3d_ttile(s, pts, g->storage, cfg, st);
             } else if (plan.path >= 1) {
                 nb = tuned3d_blocks(s, g->storage, one);
@@ -514,15 +514,15 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
             plan.path = 2;
         }
         if (cfg.time_tile > 2 && ndim == 3) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 2");
-        // two fused sweeps per launch run the f32 7-point Jacobi; for any
-        // other 3D stencil or storage the depth is a hint and sweeps run singly
-        if (ndim == 3 && cfg.time_tile == 2 && tunable && g->storage == PSKEL_DTYPE_F32) {
+        // two fused sweeps per launch run the 7-point Jacobi; for any other 3D
+        // stencil the depth is a hint and sweeps run singly
+        if (ndim == 3 && cfg.time_tile == 2 && tunable) {
             if (cfg.residual_every > 0 && cfg.residual_every < cfg.time_tile)
                 return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
             // the fused kernel addresses a plane through a buffer of < 2^31 bytes
-            if ((uint64_t)g->W * g->pitch * sizeof(float) > 0x7fffffffull)
+            if ((uint64_t)g->W * g->pitch * g->elem > 0x7fffffffull)
                 return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal tiles need planes under 2 GiB");
-            if (!ttile3d_shape_ok(cfg))
+            if (!ttile3d_shape_ok(cfg, g->storage))
                 return fail(PSKEL_ERR_UNSUPPORTED,
                             "3D temporal tile shape (cells per lane, rows per wave, waves along d and w)");
             plan.path = 2;
@@ -827,7 +827,7 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
             pskel_launch_cfg_t one = cfg;
             if (plan.path == 2) one.tile_x = one.tile_y = one.tile_z = one.block_x = one.block_y = 0;
             nb = std::max(nb, plan.path >= 1 ? tuned3d_blocks(s3, g->storage, one) : plain3d_blocks(s3, bx, by));
-            if (plan.path == 2) nb = std::max(nb, ttile3d_blocks(s3, cfg));
+            if (plan.path == 2) nb = std::max(nb, ttile3d_blocks(s3, g->storage, cfg));
         }
         if ((st = ensure_partials(g, nb)) != PSKEL_OK) return st;
         if ((st = ensure_res(g, nres_max)) != PSKEL_OK) return st;
@@ -836,7 +836,7 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
         // the fused kernel sizes its grid from device queries: make them here,
         // outside any stream capture below
         Sweep3D s3 = make_sweep3d(g, nullptr, nullptr, radius, nullptr);
-        (void)ttile3d_blocks(s3, cfg);
+        (void)ttile3d_blocks(s3, g->storage, cfg);
     }
     if (plan.path == 2 && ndim == 2 && !g->buf[2]) {
         PSKEL_HIP(hipMalloc(&g->buf[2], g->bytes));

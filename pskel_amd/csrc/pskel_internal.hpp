@@ -161,12 +161,12 @@ hipError_t launch_3d_tuned(const Sweep3D& s, const Points& p, int storage,
                            const pskel_launch_cfg_t& cfg, hipStream_t st);
 size_t plain3d_blocks(const Sweep3D& s, int bx, int by);
 size_t tuned3d_blocks(const Sweep3D& s, int storage, const pskel_launch_cfg_t& cfg);
-// two fused sweeps per launch (f32, 7-point scaled sum); cfg's tile fields
-// describe the fused kernel's tile
+// two fused sweeps per launch (7-point scaled sum, any storage); cfg's tile
+// fields describe the fused kernel's tile
 hipError_t launch_3d_ttile(const Sweep3D& s, const Points& p, int storage, const pskel_launch_cfg_t& cfg,
                            hipStream_t st);
-size_t ttile3d_blocks(const Sweep3D& s, const pskel_launch_cfg_t& cfg);
-bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg);  // an instantiated fused tile shape
+size_t ttile3d_blocks(const Sweep3D& s, int storage, const pskel_launch_cfg_t& cfg);
+bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg, int storage);  // an instantiated fused tile shape
 
 // Deterministic second-stage reduction: out = sqrt(sum(partials[0..n))).
 hipError_t launch_reduce_partials(const double* partials, size_t n, double* out, hipStream_t st);

@@ -1,8 +1,10 @@
-"""Two fused 3D sweeps per launch (time_tile=2, f32 7-point Jacobi) against the
-CPU oracle, bit-exact: the fused kernel evaluates both sweeps with the single
-sweep's expression, so any split of N sweeps into launches gives the same
-words.  Residuals of fused launches compare ||u^k - u^(k-1)|| against the
-oracle's to 1e-12 relative (different summation order)."""
+"""Two fused 3D sweeps per launch (time_tile=2, 7-point Jacobi) against the CPU
+oracle, bit-exact: the fused kernel evaluates both sweeps with the single
+sweep's expression, and with 16-bit storage rounds the first sweep to storage
+(nearest, or stochastically with that sweep's own words) before the second
+reads it, so any split of N sweeps into launches gives the same words.
+Residuals of fused launches compare ||u^k - u^(k-1)|| against the oracle's to
+1e-12 relative (different summation order)."""
 import dataclasses
 
 import numpy as np
@@ -145,8 +147,8 @@ def test_fused_rejects(gpu):
 
 
 def test_fused_depth_is_a_hint(gpu):
-    """A 3D stencil or storage the fused kernel does not run takes time_tile=2
-    as a hint and sweeps singly, with the same result."""
+    """A 3D stencil the fused kernel does not run takes time_tile=2 as a hint
+    and sweeps singly, with the same result (bf16 storage runs fused)."""
     a = _field((12, 13, 70), 15)
     halo = dataclasses.replace(K.jacobi3d(), boundary=K.BOUNDARY_HALO, halo_value=0.5)
     with api.Grid(a.shape, "f32") as g:
@@ -164,3 +166,84 @@ def test_fused_depth_is_a_hint(gpu):
         got = g.download(raw=True)
     want16, _ = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, w, 5)
     assert np.array_equal(got, want16)
+
+
+SHAPES_16 = [(4, 3, 2, 6, 1), (4, 2, 2, 8, 1)]
+
+
+def _run16(words, iters, store, tshape, per=0, offset=0.5, rounding="nearest", seed=0, **kw):
+    vd, rw, nwd, nww, pf = tshape
+    with api.Grid(words.shape, store, store_offset=offset, rounding=rounding, seed=seed) as g:
+        g.upload(0, words)
+        g.upload(1, words)
+        st = g.run(K.jacobi3d(), iters, variant="tuned", block=_block(nwd, nww, pf), tile=(vd, rw, per),
+                   time_tile=2, **kw)
+        return g.download(raw=True), st
+
+
+@pytest.mark.parametrize("tshape", SHAPES_16, ids=["vd{}_rw{}_d{}_w{}_pf{}".format(*t) for t in SHAPES_16])
+@pytest.mark.parametrize("store", ["bf16", "f16"])
+@pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
+@pytest.mark.parametrize("grid", [(3, 3, 3), (17, 9, 70), (20, 70, 300), (9, 23, 1100)],
+                         ids=["3x3x3", "17x9x70", "20x70x300", "9x23x1100"])
+def test_fused_16bit_matches_oracle(gpu, tshape, store, rounding, grid):
+    code = oracle.STORE_BF16 if store == "bf16" else oracle.STORE_F16
+    rnd = oracle.ROUND_STOCHASTIC if rounding == "stochastic" else oracle.ROUND_NEAREST
+    a = _field(grid, sum(grid) + 3)
+    for offset in (0.0, 0.5):
+        words = oracle.encode16(a, code, offset)
+        for per in (3, 0):
+            for iters in (2, 5):
+                got, st = _run16(words, iters, store, tshape, per, offset, rounding, seed=41)
+                want, _ = oracle.run_3d_r16(K.jacobi3d(), code, words, iters, store_offset=offset,
+                                            rounding=rnd, seed=41)
+                assert np.array_equal(got, want), (tshape, store, rounding, grid, offset, per, iters)
+                assert st["launches"] == (iters + 1) // 2
+
+
+@pytest.mark.parametrize("store", ["bf16", "f16"])
+def test_fused_16bit_split_runs_residuals_and_graph(gpu, store):
+    """Stochastic rounding continues its sweep index across runs and graph
+    replays; residuals of fused launches match the oracle's."""
+    code = oracle.STORE_BF16 if store == "bf16" else oracle.STORE_F16
+    shape = (30, 50, 200)
+    a = _field(shape, 16)
+    words = oracle.encode16(a, code, 0.5)
+    want7, _ = oracle.run_3d_r16(K.jacobi3d(), code, words, 7, store_offset=0.5,
+                                 rounding=oracle.ROUND_STOCHASTIC, seed=5)
+    want14, _ = oracle.run_3d_r16(K.jacobi3d(), code, words, 14, store_offset=0.5,
+                                  rounding=oracle.ROUND_STOCHASTIC, seed=5)
+    with api.Grid(shape, store, store_offset=0.5, rounding="stochastic", seed=5) as g:
+        g.upload(0, words)
+        g.upload(1, words)
+        g.run(K.jacobi3d(), 3, variant="tuned", time_tile=2)
+        g.run(K.jacobi3d(), 4, variant="tuned", time_tile=2)
+        assert np.array_equal(g.download(raw=True), want7)
+        g.run(K.jacobi3d(), 7, variant="tuned", time_tile=2, use_graph=True)
+        assert np.array_equal(g.download(raw=True), want14)
+    _, want_res = oracle.run_3d_r16(K.jacobi3d(), code, words, 20, store_offset=0.5,
+                                    rounding=oracle.ROUND_STOCHASTIC, seed=5, residual_every=4)
+    with api.Grid(shape, store, store_offset=0.5, rounding="stochastic", seed=5) as g:
+        g.upload(0, words)
+        g.upload(1, words)
+        st = g.run(K.jacobi3d(), 20, variant="tuned", time_tile=2, residual_every=4)
+    np.testing.assert_allclose(np.array(st["residuals"]), want_res, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("store", ["f32", "bf16"])
+def test_fused_host_staged(gpu, store):
+    """Host-staged slabs run fused pairs too; stochastic rounding uses the
+    slab's global plane origin."""
+    shape = (40, 24, 130)
+    a = _field(shape, 17)
+    if store == "f32":
+        want, _ = oracle.run_3d(K.jacobi3d(), a, 10)
+        got, _ = api.run_host_tiled(K.jacobi3d(), a, 10, 9, 4, time_tile=2)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    else:
+        words = oracle.encode16(a, oracle.STORE_BF16, 0.5)
+        want, _ = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, words, 10, store_offset=0.5,
+                                    rounding=oracle.ROUND_STOCHASTIC, seed=77)
+        got, _ = api.run_host_tiled(K.jacobi3d(), words, 10, 9, 4, storage="bf16", store_offset=0.5,
+                                    rounding="stochastic", seed=77, time_tile=2)
+        assert np.array_equal(got, want)
