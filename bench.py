@@ -56,7 +56,8 @@ TUNED = {
     "jacobi3d_single": dict(tile=(4, 1, 128)),
     "jacobi3d_bf16": dict(tile=(2, 2, 32)),
     "jacobi3d_f16": dict(tile=(4, 2, 32)),
-    "life": dict(tile=(0, 16, 0), time_tile=4),
+    # 8 generations per launch on bit-packed tiles of 32 rows per lane
+    "life": dict(tile=(0, 32, 0), time_tile=8),
     "jacobi2d": dict(tile=(0, 16, 0), time_tile=4),
     "diffusion9": dict(tile=(0, 16, 0), time_tile=3),
     "cloudsim": dict(tile=(0, 1, 2)),

@@ -259,9 +259,12 @@ public:
         }
         pskel_launch_cfg_t cfg{};
         cfg.variant = PSKEL_VARIANT_TUNED;
-        // temporal depth: up to 4 sweeps per launch in 2D, 2 in 3D (the 7-point
-        // Jacobi in f32; other 3D stencils sweep singly)
-        cfg.time_tile = (int)std::max<size_t>(1, std::min<size_t>(innerIterations, dims() == 2 ? 4 : 2));
+        // temporal depth: up to 2 sweeps per launch in 3D (the 7-point Jacobi;
+        // other 3D stencils sweep singly), 4 in 2D, 8 for life
+        size_t kmax = dims() == 2 ? 4 : 2;
+        if constexpr (StencilTraits<Args>::has_descriptor)
+            if (dims() == 2 && StencilTraits<Args>::describe(mask, args, 2).kind == PSKEL_KIND_LIFE) kmax = 8;
+        cfg.time_tile = (int)std::max<size_t>(1, std::min<size_t>(innerIterations, kmax));
         gpuRun(iterations, cfg, false);
     }
 

@@ -103,6 +103,21 @@ struct Stor<PSKEL_DTYPE_F16> {
     }
 };
 
+// Raw buffer resources.  The hardware range check turns a load past the
+// resource's size into zeros and drops such a store, so kernels can issue the
+// same memory operations on every path (exact wait counts) and mask with an
+// offset past the data, or a resource of 0 bytes.
+using u32x4_t = __attribute__((ext_vector_type(4))) unsigned int;
+using u32x2_t = __attribute__((ext_vector_type(2))) unsigned int;
+
+// a raw buffer over `bytes` bytes from p (0 bytes: every access out of range)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* p, int bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+
+constexpr uint32_t kNoAccess = 0x80000000u;  // an offset past any buffer (all buffers stay under 2 GiB)
+
 // Vector load/store of N consecutive elements of T (N * sizeof(T) in {2,4,8,16} bytes).
 template <typename T, int N>
 struct Vec {

@@ -98,17 +98,6 @@ size_t march3d_t2_f16_blocks(const Sweep3D& s, const March2Shape& m);
 #ifdef PSKEL_MARCH_INSTANTIATE
 namespace {
 
-using u32x4_t = __attribute__((ext_vector_type(4))) unsigned int;
-using u32x2_t = __attribute__((ext_vector_type(2))) unsigned int;
-
-// a raw buffer over `bytes` bytes from p (0 bytes: every access out of range)
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* p, int bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
-}
-
-constexpr uint32_t kNoAccess = 0x80000000u;  // an offset past any plane
-
 template <int VD>
 struct LVec;  // one lane's vector as an LDS word
 template <>

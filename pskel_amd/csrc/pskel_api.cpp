@@ -398,7 +398,7 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
                 if (plan.path >= 1) e = launch_cloud_tuned(s, ca, one, st);
                 else e = launch_cloud_plain(s, ca, bx, by, st);
             } else if (g->storage == PSKEL_DTYPE_U8) {
-                if (k > 1) e = launch_life_ttile(s, (int)k, cfg, st);
+                if (k > 1) e = launch_life_bits(s, (int)k, cfg, st);
                 else if (plan.path >= 1) e = launch_life_tuned(s, one, st);
                 else e = launch_life_plain(s, pts, bx, by, st);
             } else {
@@ -508,7 +508,9 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
         if (tunable) plan.path = 1;
         // field stencils run one sweep per launch (no temporal tiles)
         if (tunable && ndim == 2 && cfg.time_tile > 1 && plan.pattern != PAT_CLOUD) {
-            if (cfg.time_tile > 4) return fail(PSKEL_ERR_UNSUPPORTED, "temporal depth above 4");
+            // life runs up to 8 generations per launch on bit-packed cells
+            const int kmax = plan.pattern == PAT_LIFE8 ? 8 : 4;
+            if (cfg.time_tile > kmax) return fail(PSKEL_ERR_UNSUPPORTED, "temporal depth above 4 (8 for life)");
             if (cfg.residual_every > 0 && cfg.residual_every < cfg.time_tile)
                 return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
             plan.path = 2;

@@ -152,7 +152,7 @@ hipError_t launch_2d_ttile_f32(const Sweep2D& s, const Points& p, int pattern, i
 // 2D life (u8) sweeps.
 hipError_t launch_life_plain(const Sweep2D& s, const Points& p, int bx, int by, hipStream_t st);
 hipError_t launch_life_tuned(const Sweep2D& s, const pskel_launch_cfg_t& cfg, hipStream_t st);
-hipError_t launch_life_ttile(const Sweep2D& s, int k, const pskel_launch_cfg_t& cfg, hipStream_t st);
+hipError_t launch_life_bits(const Sweep2D& s, int k, const pskel_launch_cfg_t& cfg, hipStream_t st);
 
 // 3D sweeps for every storage type.
 hipError_t launch_3d_plain(const Sweep3D& s, const Points& p, int storage, int bx, int by,

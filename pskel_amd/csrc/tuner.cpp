@@ -81,9 +81,13 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
     if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9 || pat == PAT_LIFE8 || pat == PAT_CLOUD)) {
         const int ry_nwy[][2] = {{1, 2}, {1, 4}, {2, 2}, {2, 4}, {4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}};
         for (auto& t : ry_nwy) v.push_back(tuned(0, t[0], t[1], 1));
-        if (pat != PAT_CLOUD)  // field stencils run one sweep per launch
+        if (pat == PAT_LIFE8) {  // bit-packed tiles: rows per lane x generations per launch
+            for (int k : {2, 4, 6, 8})
+                for (int rows : {32, 40, 48}) v.push_back(tuned(0, rows, 0, k));
+        } else if (pat != PAT_CLOUD) {  // field stencils run one sweep per launch
             for (int k = 2; k <= 4; ++k)
                 for (int cy : {16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
+        }
         v.push_back(plain(64, 4));
     } else if (q.ndim == 3 && pat == PAT_J7) {
         const bool narrow = q.storage != PSKEL_DTYPE_F32;
@@ -152,6 +156,7 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
     if (q.ndim == 2 && pat == PAT_CLOUD) return tuned(0, 2, 4, 1);
     if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9 || pat == PAT_LIFE8)) {
         const double cells = (double)q.width * q.height;
+        if (pat == PAT_LIFE8 && q.iterations >= 8 && cells >= (1 << 20)) return tuned(0, 32, 0, 8);
         if (q.iterations >= 4 && cells >= (1 << 20)) return tuned(0, 16, 0, pat == PAT_BOX9 ? 3 : 4);
         return tuned(0, 2, 4, 1);
     }

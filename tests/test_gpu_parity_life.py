@@ -38,9 +38,10 @@ def test_life_tuned_shapes(gpu, ry, nwy):
     assert np.array_equal(_run(b, 5, variant="tuned", tile=(0, ry, nwy)), oracle.run_2d(K.life(), b, 5))
 
 
-@pytest.mark.parametrize("k", [2, 3, 4])
-@pytest.mark.parametrize("cy", [16, 32, 64])
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("cy", [16, 32, 48, 64])
 def test_life_temporal(gpu, k, cy):
+    """k generations per launch on bit-packed tiles of cy (rounded) rows per lane."""
     b = inputs.life_field(500, seed=4)
     for iters in (k, 2 * k + 1, 11):
         got = _run(b, iters, variant="tuned", time_tile=k, tile=(0, cy, 0))
@@ -53,10 +54,25 @@ def test_life_distinct_rings(gpu):
     o[0, ::3] = 1
     o[-1, ::2] = 1
     o[:, 0] = 1
+    o[:, -1] = 1
+    o[5:9, -1] = 0
     for kw in ({"variant": "plain"}, {"variant": "tuned"}, {"variant": "tuned", "time_tile": 2},
-               {"variant": "tuned", "time_tile": 3}):
-        for iters in (1, 2, 5, 6):
+               {"variant": "tuned", "time_tile": 3}, {"variant": "tuned", "time_tile": 7},
+               {"variant": "tuned", "time_tile": 8}):
+        for iters in (1, 2, 5, 6, 15, 16):
             assert np.array_equal(_run(b, iters, out=o, **kw), oracle.run_2d(K.life(), b, iters, out=o)), (kw, iters)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 40), (3, 3), (40, 2), (33, 33), (70, 1985), (41, 1986),
+                                   (130, 4000), (2, 4100), (4100, 3)])
+@pytest.mark.parametrize("k", [2, 5, 8])
+def test_life_temporal_ragged(gpu, shape, k):
+    """Tile, lane and word boundaries against the grid's edges, distinct rings."""
+    b = (inputs.uniform(31, 0, shape[0] * shape[1]).reshape(shape) < 0.4).astype(np.uint8)
+    o = (inputs.uniform(32, 0, shape[0] * shape[1]).reshape(shape) < 0.5).astype(np.uint8)
+    for iters in (k, k + 3):
+        got = _run(b, iters, out=o, variant="tuned", time_tile=k)
+        assert np.array_equal(got, oracle.run_2d(K.life(), b, iters, out=o)), (shape, k, iters)
 
 
 def test_life_rejects_non_binary_cells(gpu):
@@ -73,5 +89,6 @@ def test_life_full_size(gpu):
     """8192^2 board (8194^2 with the dead ring) x 40 generations, all variants."""
     b = inputs.life_field(8192, seed=2)
     want = oracle.run_2d(K.life(), b, 40)
-    for kw in ({"variant": "plain", "block": (64, 4)}, {"variant": "tuned"}, {"variant": "tuned", "time_tile": 4}):
+    for kw in ({"variant": "plain", "block": (64, 4)}, {"variant": "tuned"}, {"variant": "tuned", "time_tile": 4},
+               {"variant": "tuned", "time_tile": 8, "tile": (0, 48, 0)}):
         assert np.array_equal(_run(b, 40, **kw), want), kw
