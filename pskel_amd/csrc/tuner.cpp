@@ -156,7 +156,10 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
     if (q.ndim == 2 && pat == PAT_CLOUD) return tuned(0, 2, 4, 1);
     if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9 || pat == PAT_LIFE8)) {
         const double cells = (double)q.width * q.height;
-        if (pat == PAT_LIFE8 && q.iterations >= 8 && cells >= (1 << 20)) return tuned(0, 32, 0, 8);
+        // bit-packed tiles are 1984 columns wide: narrower grids leave lanes idle
+        if (pat == PAT_LIFE8 && q.iterations >= 8 && cells >= (1 << 20) && q.width >= 2048)
+            return tuned(0, 32, 0, 8);
+        if (pat == PAT_LIFE8) return tuned(0, 2, 4, 1);
         if (q.iterations >= 4 && cells >= (1 << 20)) return tuned(0, 16, 0, pat == PAT_BOX9 ? 3 : 4);
         return tuned(0, 2, 4, 1);
     }
