@@ -109,25 +109,32 @@ uint32_t oracle_sr_key(uint32_t seed, uint64_t sweep)
     return mix32(seed + 0x9e3779b9u * ((uint32_t)sweep + 1u));
 }
 
-/* Random word of cell (h, w, d) in the sweep with key `key`: A(w*D + d) ^ B(h),
- * two independent hash streams of the in-plane index and of the plane.  For
- * two distinct cells the words are pairwise independent (distinct in-plane
- * index: independent A terms; same one: the B terms differ), so the rounding
- * errors are uncorrelated as with a fresh draw per cell. */
+/* Random word of cell (h, w, d) in sweep t: A(w*D + d) ^ B(t, h), two
+ * independent hash streams: A of the in-plane index under a key of the seed
+ * alone (oracle_sr_cell_key), B of the plane under the sweep's key
+ * (oracle_sr_key).  Two distinct (cell, sweep) pairs get pairwise independent
+ * words (distinct in-plane index: independent A terms; the same one: the B
+ * terms differ), so the rounding errors of a linear sweep are uncorrelated as
+ * with a fresh draw per cell and sweep. */
+uint32_t oracle_sr_cell_key(uint32_t seed)
+{
+    return mix32(mix32(seed) ^ 0xa511e9b3u);
+}
+
 uint32_t oracle_sr_plane_bits(uint32_t key, uint64_t h)
 {
     return mix32(mix32(key + 0x632be5abu) ^ (uint32_t)h);
 }
 
-uint32_t oracle_sr_inplane_bits(uint32_t key, uint64_t wd)
+uint32_t oracle_sr_cell_bits(uint32_t cell_key, uint64_t wd)
 {
-    return mix32(key ^ (uint32_t)wd);
+    return mix32(cell_key ^ (uint32_t)wd);
 }
 
 uint32_t oracle_sr_bits(uint32_t seed, uint64_t sweep, uint64_t h, uint64_t wd)
 {
-    const uint32_t key = oracle_sr_key(seed, sweep);
-    return oracle_sr_inplane_bits(key, wd) ^ oracle_sr_plane_bits(key, h);
+    return oracle_sr_cell_bits(oracle_sr_cell_key(seed), wd) ^
+           oracle_sr_plane_bits(oracle_sr_key(seed, sweep), h);
 }
 
 /* bf16: add the top 16 bits of r below the kept bits and truncate, so the
@@ -454,6 +461,7 @@ static double sweep3d_r16(const plan3d* p, int store, float off, const uint16_t*
     const enc_fn enc = store == ORACLE_STORE_BF16 ? oracle_f32_to_bf16 : oracle_f32_to_f16;
     const size_t r = p->r;
     const uint32_t key = oracle_sr_key(seed, sweep);
+    const uint32_t cell_key = oracle_sr_cell_key(seed);
     double* rows = want_res ? (double*)calloc(H, sizeof(double)) : NULL;
 #pragma omp parallel for num_threads(th) schedule(static)
     for (size_t h = 0; h < H; ++h) {
@@ -480,7 +488,7 @@ static double sweep3d_r16(const plan3d* p, int store, float off, const uint16_t*
                     }
                     if (p->kind == ORACLE_KIND_SCALED_SUM) acc = p->cf * acc;
                     if (rounding == ORACLE_ROUND_STOCHASTIC) {
-                        const uint32_t rb = oracle_sr_inplane_bits(key, w * D + dd) ^ plane_bits;
+                        const uint32_t rb = oracle_sr_cell_bits(cell_key, w * D + dd) ^ plane_bits;
                         sv = store == ORACLE_STORE_BF16 ? oracle_f32_to_bf16_sr(acc - off, rb)
                                                         : oracle_f32_to_f16_sr(acc - off, rb);
                     } else {

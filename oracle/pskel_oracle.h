@@ -58,8 +58,9 @@ enum { ORACLE_STORE_F32 = 0, ORACLE_STORE_BF16 = 1, ORACLE_STORE_F16 = 2 };
  * whose rounding they restate. */
 enum { ORACLE_ROUND_NEAREST = 0, ORACLE_ROUND_STOCHASTIC = 1 };
 uint32_t oracle_sr_key(uint32_t seed, uint64_t sweep);
+uint32_t oracle_sr_cell_key(uint32_t seed);
 uint32_t oracle_sr_plane_bits(uint32_t key, uint64_t h);
-uint32_t oracle_sr_inplane_bits(uint32_t key, uint64_t wd);
+uint32_t oracle_sr_cell_bits(uint32_t cell_key, uint64_t wd);
 uint32_t oracle_sr_bits(uint32_t seed, uint64_t sweep, uint64_t h, uint64_t wd);
 uint16_t oracle_f32_to_bf16_sr(float x, uint32_t r);
 uint16_t oracle_f32_to_f16_sr(float x, uint32_t r);

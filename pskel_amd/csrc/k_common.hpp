@@ -13,12 +13,15 @@ namespace pskel {
 constexpr int kWave = 64;
 
 // ---------------------------------------------------------------------------
-// Stochastic rounding words.  mix32 is a murmur3-style avalanche.  A sweep's
-// key mixes the seed with the sweep index; the 32-bit random word of cell
-// (h, w, d) is A(w*D + d) ^ B(h), two independent hash streams of the
-// in-plane index and of the plane: pairwise independent between distinct
-// cells, and a kernel marching along h hashes A once per cell of its tile and
-// B once per plane (a scalar).
+// Stochastic rounding words.  mix32 is a murmur3-style avalanche.  The 32-bit
+// random word of cell (h, w, d) in sweep t is A(w*D + d) ^ B(t, h), two
+// independent hash streams: A of the in-plane index under a key of the seed
+// alone, B of the plane under the sweep's key (seed and sweep index).  Two
+// distinct (cell, sweep) pairs get pairwise independent words (distinct
+// in-plane index: independent A terms; the same one: the B terms differ), so
+// the rounding errors of a linear sweep are uncorrelated.  A kernel marching
+// along h hashes A once per cell it owns, for all its planes and sweeps, and B
+// once per plane (a scalar).
 // ---------------------------------------------------------------------------
 __host__ __device__ __forceinline__ uint32_t mix32(uint32_t h)
 {
@@ -35,14 +38,21 @@ __host__ __device__ __forceinline__ uint32_t sweep_key(uint32_t seed, uint64_t s
     return mix32(seed + 0x9e3779b9u * ((uint32_t)sweep + 1u));
 }
 
+__host__ __device__ __forceinline__ uint32_t sr_cell_key(uint32_t seed)
+{
+    return mix32(mix32(seed) ^ 0xa511e9b3u);
+}
+
+// B(t, h): `key` is sweep_key(seed, t)
 __host__ __device__ __forceinline__ uint32_t sr_plane_bits(uint32_t key, uint32_t h)
 {
     return mix32(mix32(key + 0x632be5abu) ^ h);
 }
 
-__host__ __device__ __forceinline__ uint32_t sr_inplane_bits(uint32_t key, uint32_t wd)
+// A(wd): `cell_key` is sr_cell_key(seed)
+__host__ __device__ __forceinline__ uint32_t sr_cell_bits(uint32_t cell_key, uint32_t wd)
 {
-    return mix32(key ^ wd);
+    return mix32(cell_key ^ wd);
 }
 
 // ---------------------------------------------------------------------------

@@ -217,10 +217,11 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
     const uint32_t key = SR ? sweep_key(s.seed, s.sweep) : 0u;
     uint32_t ra[RW][VD];
     if constexpr (SR) {
+        const uint32_t ckey = sr_cell_key(s.seed);
 #pragma unroll
         for (int i = 0; i < RW; ++i)
 #pragma unroll
-            for (int j = 0; j < VD; ++j) ra[i][j] = sr_inplane_bits(key, (uint32_t)(wr + i) * D + d + j);
+            for (int j = 0; j < VD; ++j) ra[i][j] = sr_cell_bits(ckey, (uint32_t)(wr + i) * D + d + j);
     }
     double rr = 0.0;
     int b = 0;

@@ -40,6 +40,8 @@
 // plane h+1 never drains the planes prefetched after it.
 #pragma once
 
+#include <atomic>
+
 #include "k_common.hpp"
 
 namespace pskel {
@@ -239,6 +241,17 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
                                     ? (uint32_t)(((size_t)hrow * pitch + d) * sizeof(T))
                                     : kNoAccess;
 
+        // stochastic rounding: the in-plane words A of this lane's cells,
+        // hashed once and shared by every plane and both sweeps
+        uint32_t ra[RW][VD];
+        if constexpr (SR) {
+            const uint32_t ckey = sr_cell_key(s.seed);
+#pragma unroll
+            for (int i = 0; i < RW; ++i)
+#pragma unroll
+                for (int j = 0; j < VD; ++j) ra[i][j] = sr_cell_bits(ckey, (uint32_t)(wr + i) * D + d + j);
+        }
+
         constexpr int NQ = PF + 3;  // ring slots: planes h-1 .. h+PF+1
         T in[NQ][RW][VD];           // input planes (storage words)
         T hal[NQ][VD];              // input planes of the halo row
@@ -276,23 +289,14 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         // With HALO, L carries the halo rows (the input plane); without, the
         // tile's edge rows take a stand-in neighbour (those cells are never output).
         // out: storage words; a face keeps its word `midw`, any other cell rounds
-        // coeff * sum (stochastically: with the word of its sweep `key`, plane
-        // and cell, hashed here, where issue slots are spare and registers not)
+        // coeff * sum (stochastically: with the word ra ^ B(sweep, plane), where
+        // `key` is the sweep's key)
         auto sweep = [&](auto halo, int p, const float (&lo)[RW][VD], const float (&mid)[RW][VD],
                          const float (&hi)[RW][VD], const T (&midw)[RW][VD], const LV (*L)[NL],
                          const float (*E)[NWD][RW][2], uint32_t key, T (&out)[RW][VD]) {
             constexpr bool HALO = decltype(halo)::value;
             const bool pint = p >= r && p < H - r;
             const uint32_t rb = SR ? sr_plane_bits(key, s.plane0 + (uint32_t)p) : 0u;  // uniform
-            // The in-plane hashes do not change along h; opaque copies of the
-            // key and of this lane's first cell index keep the compiler from
-            // hoisting all RW x VD of them into registers (an empty asm, no
-            // instruction).
-            uint32_t kk = key, cell0 = (uint32_t)wr * (uint32_t)D + (uint32_t)d;
-            if constexpr (SR) {
-                asm volatile("" : "+s"(kk));
-                asm volatile("" : "+v"(cell0));
-            }
 #pragma unroll
             for (int i = 0; i < RW; ++i) {
                 float wm[VD], wp[VD];
@@ -330,8 +334,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
                     const bool face = face_row || (!dint && (d + j < r || d + j >= D - r));
                     T v;
                     if constexpr (SR) {
-                        const uint32_t wd_idx = cell0 + (uint32_t)i * (uint32_t)D + (uint32_t)j;
-                        v = Tr::enc_sr(coeff * acc, off, sr_inplane_bits(kk, wd_idx) ^ rb);
+                        v = Tr::enc_sr(coeff * acc, off, ra[i][j] ^ rb);
                     } else {
                         v = Tr::enc(coeff * acc, off);
                     }
@@ -453,7 +456,9 @@ size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
     int q = m.hseg > 0 ? std::min(m.hseg, s.H) : 0;
     if (q <= 0) {
         // resident blocks per CU of this kernel: the least over its variants
-        static int occ = 0;
+        // (queried once; grids may launch from several host threads)
+        static std::atomic<int> occ_cache{0};
+        int occ = occ_cache.load(std::memory_order_relaxed);
         if (occ == 0) {
             constexpr int NT = NWD * NWW * kWave;
             int o[4] = {1, 1, 1, 1};
@@ -468,16 +473,18 @@ size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
                      hipOccupancyMaxActiveBlocksPerMultiprocessor(
                          &o[3], k3d_march2<ST, VD, RW, NWD, NWW, PF, false, true>, NT, 0) == hipSuccess;
             occ = ok ? std::max(1, std::min(std::min(o[0], o[1]), std::min(o[2], o[3]))) : 1;
+            occ_cache.store(occ, std::memory_order_relaxed);
         }
-        static int cus_of[64] = {};  // CUs per device, queried once
+        static std::atomic<int> cus_of[64] = {};  // CUs per device, queried once
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-        if (cus_of[dev] <= 0) {
-            int c = 0;
-            if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 1;
-            cus_of[dev] = c;
+        int cus = cus_of[dev].load(std::memory_order_relaxed);
+        if (cus <= 0) {
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+                cus = 1;
+            cus_of[dev].store(cus, std::memory_order_relaxed);
         }
-        const int64_t slots = (int64_t)cus_of[dev] * occ;
+        const int64_t slots = (int64_t)cus * occ;
         double best = -1.0;
         for (int n = 1; n <= std::min(s.H, 256); ++n) {
             const int hs = (s.H + n - 1) / n;

@@ -68,7 +68,8 @@ __global__ __launch_bounds__(1024) void k3d_plain(Sweep3D s, Points p)
             if (s.rounding == PSKEL_ROUND_STOCHASTIC)
                 {
                 const uint32_t key = sweep_key(s.seed, s.sweep);
-                res = Tr::enc_sr(acc, off, sr_inplane_bits(key, (uint32_t)w * D + d) ^ sr_plane_bits(key, s.plane0 + h));
+                res = Tr::enc_sr(acc, off, sr_cell_bits(sr_cell_key(s.seed), (uint32_t)w * D + d) ^
+                                               sr_plane_bits(key, s.plane0 + h));
             }
             else
                 res = Tr::enc(acc, off);

@@ -251,30 +251,44 @@ def test_stochastic_matches_recorded_gfx950_conversions():
 
 
 def test_sr_bits_statistics():
-    """Random words A(w*D + d) ^ B(h): the top 16 bits close to uniform per bit
-    along a plane and along a column; pairwise decorrelated between cells of a
-    plane, cells of a column, consecutive sweeps and seeds.  Two cells of one
-    column in different planes are related within a sweep by the fixed word
-    B(h) ^ B(h'), which is itself uniform over sweeps: over many sweeps their
-    words are uncorrelated."""
+    """Random words A(w*D + d) ^ B(t, h): the top 16 bits close to uniform per
+    bit along a plane, a column and the sweeps of one cell; pairwise
+    decorrelated between neighbouring cells of a plane, of a column, between
+    consecutive sweeps of one cell and between seeds.  Words that share one of
+    the two terms are related by the XOR of the other two (two planes of a
+    sweep, or one plane in two sweeps, by B ^ B' over every cell; two cells by
+    A ^ A' over every plane and sweep), and that relating word is itself
+    uniform over the pairs: B(t, h) ^ B(t', h) over t, A(c) ^ A(c + 1) over
+    c."""
     n = 1 << 14
     top = lambda a: (a >> np.uint64(16)).astype(np.float64) / 2 ** 16
+    bits = lambda a: np.array([((a >> np.uint64(k)) & np.uint64(1)).mean() for k in range(16, 32)])
     plane = np.array([oracle.sr_bits(11, 0, 5, c) for c in range(n)], np.uint64)
     column = np.array([oracle.sr_bits(11, 0, h, 77) for h in range(n)], np.uint64)
-    nxt = np.array([oracle.sr_bits(11, 1, 5, c) for c in range(n)], np.uint64)
+    sweeps = np.array([oracle.sr_bits(11, t, 5, 77) for t in range(n)], np.uint64)
     seed2 = np.array([oracle.sr_bits(12, 0, 5, c) for c in range(n)], np.uint64)
-    for a in (plane, column):
-        freq = np.array([((a >> np.uint64(k)) & np.uint64(1)).mean() for k in range(16, 32)])
-        assert np.all(np.abs(freq - 0.5) < 5 * 0.5 / np.sqrt(n))
+    lim5 = 5 * 0.5 / np.sqrt(n)
+    for a in (plane, column, sweeps):
+        assert np.all(np.abs(bits(a) - 0.5) < lim5)
     lim = 5 / np.sqrt(n)
-    fp, fc = top(plane), top(column)
+    fp, fc, fs = top(plane), top(column), top(sweeps)
     assert abs(np.corrcoef(fp[:-1], fp[1:])[0, 1]) < lim
     assert abs(np.corrcoef(fc[:-1], fc[1:])[0, 1]) < lim
-    assert abs(np.corrcoef(fp, top(nxt))[0, 1]) < lim
+    assert abs(np.corrcoef(fs[:-1], fs[1:])[0, 1]) < lim
     assert abs(np.corrcoef(fp, top(seed2))[0, 1]) < lim
-    h5 = np.array([oracle.sr_bits(11, s, 5, 77) for s in range(n)], np.uint64)
-    h6 = np.array([oracle.sr_bits(11, s, 6, 77) for s in range(n)], np.uint64)
-    assert abs(np.corrcoef(top(h5), top(h6))[0, 1]) < lim
+    h6 = np.array([oracle.sr_bits(11, t, 6, 77) for t in range(n)], np.uint64)
+    assert abs(np.corrcoef(fs, top(h6))[0, 1]) < lim
+    # the relating words: uniform over sweeps (one plane, sweeps t and t + 1)
+    # and over cells (neighbouring cells of one plane)
+    assert np.all(np.abs(bits(sweeps[:-1] ^ sweeps[1:]) - 0.5) < lim5)
+    assert np.all(np.abs(bits(plane[:-1] ^ plane[1:]) - 0.5) < lim5)
+    # the split itself: A depends on the seed and cell only, B on the seed,
+    # sweep and plane only
+    a = lambda seed, t, h, c: oracle.sr_bits(seed, t, h, c)
+    for t, h, c, c2 in ((0, 5, 77, 78), (9, 0, 3, 1000), (1 << 33, 7, 0, 12)):
+        assert a(11, t, h, c) ^ a(11, t, h, c2) == a(11, t + 1, h + 3, c) ^ a(11, t + 1, h + 3, c2)
+        assert a(11, t, h, c) ^ a(11, t + 1, h, c) == a(11, t, h, c2) ^ a(11, t + 1, h, c2)
+        assert a(11, t, h, c) ^ a(12, t, h, c) != a(11, t, h, c2) ^ a(12, t, h, c2)
 
 
 def test_r16_stochastic_sweep_is_unbiased():
