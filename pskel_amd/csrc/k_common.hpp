@@ -90,7 +90,9 @@ struct Stor<PSKEL_DTYPE_BF16> {
     }
     __device__ __forceinline__ static T enc_sr(float v, float off, uint32_t r)
     {
-        bf16x2_t o = {};
+        // the converted half only: the other half is left unspecified (no
+        // register initialisation)
+        bf16x2_t o = __builtin_nondeterministic_value(o);
         o = __builtin_amdgcn_cvt_sr_bf16_f32(o, v - off, r, false);
         return __builtin_bit_cast(T, o[0]);
     }
@@ -107,7 +109,7 @@ struct Stor<PSKEL_DTYPE_F16> {
     }
     __device__ __forceinline__ static T enc_sr(float v, float off, uint32_t r)
     {
-        f16x2_t o = {};
+        f16x2_t o = __builtin_nondeterministic_value(o);
         o = __builtin_amdgcn_cvt_sr_f16_f32(o, v - off, r, false);
         return __builtin_bit_cast(T, o[0]);
     }

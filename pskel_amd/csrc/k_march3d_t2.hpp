@@ -26,7 +26,9 @@
 // from lane shuffles.  LDS is double-buffered: one barrier per plane, and
 // consecutive lanes touch consecutive 8- or 16-byte words (no bank conflicts).
 //
-// 16-bit storage.  The register rings hold storage words and decode on use.
+// 16-bit storage.  The register rings hold storage words, two cells per 32-bit
+// register, and decode on use; results are rounded two at a time into such a
+// register, and faces are merged in by a per-register bit mask.
 // A first-sweep value is rounded to storage (to nearest, or stochastically
 // with the word of its own sweep index, plane and cell) before the second
 // sweep reads it, so the pair equals two stored sweeps bit for bit; faces
@@ -41,6 +43,7 @@
 #pragma once
 
 #include <atomic>
+#include <type_traits>
 
 #include "k_common.hpp"
 
@@ -164,11 +167,88 @@ __device__ __forceinline__ void lds_get(const typename LVec<VD>::T* p, float (&v
     __builtin_memcpy(v, &x, sizeof(x));
 }
 
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// A lane's VD cells as storage words: f32 one cell per word; bf16 and f16 two
+// cells per 32-bit word, cell 2k in the low half of word k and 2k+1 in the
+// high half (the order of memory).
+template <int ST, int VD>
+struct LaneWords {
+    static constexpr int CPW = ST == PSKEL_DTYPE_F32 ? 1 : 2;  // cells per word
+    static constexpr int N = VD / CPW;
+    using W = std::conditional_t<ST == PSKEL_DTYPE_F32, float, uint32_t>;
+
+    // cell j as stored (u - offset for 16-bit storage)
+    __device__ __forceinline__ static float raw(const W (&w)[N], int j)
+    {
+        if constexpr (ST == PSKEL_DTYPE_F32) {
+            return w[j];
+        } else if constexpr (ST == PSKEL_DTYPE_BF16) {
+            return __uint_as_float((j & 1) ? (w[j >> 1] & 0xffff0000u) : (w[j >> 1] << 16));
+        } else {
+            return __half2float(__ushort_as_half((uint16_t)((j & 1) ? (w[j >> 1] >> 16) : w[j >> 1])));
+        }
+    }
+    // cell j's value
+    __device__ __forceinline__ static float cell(const W (&w)[N], int j, float off)
+    {
+        if constexpr (ST == PSKEL_DTYPE_F32) return w[j];
+        else return raw(w, j) + off;
+    }
+    // v - off rounded to storage, to nearest even (as Stor<ST>::enc)
+    __device__ __forceinline__ static void put(W (&w)[N], const float (&v)[VD], float off)
+    {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            if constexpr (ST == PSKEL_DTYPE_F32) {
+                w[k] = v[k];
+            } else {
+                const f32x2_t f = {v[2 * k] - off, v[2 * k + 1] - off};
+                if constexpr (ST == PSKEL_DTYPE_BF16) w[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2_t));
+                else w[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, f16x2_t));
+            }
+        }
+    }
+    // v - off rounded stochastically with the random word r[j] of each cell
+    // (as Stor<ST>::enc_sr); 16-bit only
+    __device__ __forceinline__ static void put_sr(W (&w)[N], const float (&v)[VD], float off, const uint32_t (&r)[VD])
+    {
+        static_assert(ST != PSKEL_DTYPE_F32, "stochastic rounding is for 16-bit storage");
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const float a = v[2 * k] - off, b = v[2 * k + 1] - off;
+            // the first conversion replaces the low half of a's own register,
+            // the second the high half
+            if constexpr (ST == PSKEL_DTYPE_BF16) {
+                bf16x2_t o = __builtin_bit_cast(bf16x2_t, a);
+                o = __builtin_amdgcn_cvt_sr_bf16_f32(o, a, r[2 * k], false);
+                o = __builtin_amdgcn_cvt_sr_bf16_f32(o, b, r[2 * k + 1], true);
+                w[k] = __builtin_bit_cast(uint32_t, o);
+            } else {
+                f16x2_t o = __builtin_bit_cast(f16x2_t, a);
+                o = __builtin_amdgcn_cvt_sr_f16_f32(o, a, r[2 * k], false);
+                o = __builtin_amdgcn_cvt_sr_f16_f32(o, b, r[2 * k + 1], true);
+                w[k] = __builtin_bit_cast(uint32_t, o);
+            }
+        }
+    }
+    // the bits of `keep` where m is set, of `w` elsewhere
+    __device__ __forceinline__ static W merge(uint32_t m, W keep, W w)
+    {
+        const uint32_t x = (__builtin_bit_cast(uint32_t, keep) & m) | (__builtin_bit_cast(uint32_t, w) & ~m);
+        return __builtin_bit_cast(W, x);
+    }
+    // the mask of cell j within its word
+    static constexpr uint32_t cell_mask(int j) { return CPW == 1 ? ~0u : 0xffffu << (16 * (j % CPW)); }
+};
+
 template <int ST, int VD, int RW, int NWD, int NWW, int PF, bool RES, bool SR>
 __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float coeff, int hseg, int ntd, int ntw)
 {
-    using Tr = Stor<ST>;
-    using T = typename Tr::T;  // storage word
+    using T = typename Stor<ST>::T;  // storage element
+    using LW = LaneWords<ST, VD>;
+    using Word = typename LW::W;  // register word: one f32 cell or two 16-bit cells
+    constexpr int NV = LW::N;  // words per lane vector
     constexpr int NW = NWD * NWW;
     constexpr int TD = NWD * kWave * VD;
     constexpr int TW = NWW * RW;
@@ -183,7 +263,8 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
     __shared__ double red[NW];
 
     const int lane = threadIdx.x % kWave;
-    const int wave = threadIdx.x / kWave;
+    // uniform across the wave, so rows, tile edges and LDS slots are scalars
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int wd = wave % NWD, ww = wave / NWD;
     const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
 
@@ -216,6 +297,13 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         const bool dlive = d < D;
         const bool dout = dlive && d >= od0 && d < od1;
         const bool dint = d >= r && d + VD - 1 < D - r;
+        // cells of this lane on a d face, as a bit mask per register word
+        uint32_t dmask[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) dmask[k] = 0u;
+#pragma unroll
+        for (int j = 0; j < VD; ++j)
+            if (!dint && (d + j < r || d + j >= D - r)) dmask[j / LW::CPW] |= LW::cell_mask(j);
         bool row_int[RW], row_out[RW];
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
@@ -253,23 +341,23 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         }
 
         constexpr int NQ = PF + 3;  // ring slots: planes h-1 .. h+PF+1
-        T in[NQ][RW][VD];           // input planes (storage words)
-        T hal[NQ][VD];              // input planes of the halo row
-        T s1[NQ][RW][VD];           // first-sweep planes h-2, h-1, h (the rest unused), storage words
+        Word in[NQ][RW][NV];           // input planes (storage words)
+        Word hal[NQ][NV];              // input planes of the halo row
+        Word s1[NQ][RW][NV];           // first-sweep planes h-2, h-1, h (the rest unused), storage words
 
 
-        auto load = [&](int p, T (&v)[RW][VD], T (&hv)[VD]) {
+        auto load = [&](int p, Word (&v)[RW][NV], Word (&hv)[NV]) {
             const bool ok = p >= hlo && p < hhi;  // uniform across the block
             const auto rs = plane_rsrc((const T*)s.src + (size_t)(ok ? p : 0) * plane, ok ? plane_bytes : 0);
 #pragma unroll
-            for (int i = 0; i < RW; ++i) buf_load<T, VD>(rs, ld_off[i], v[i]);
-            buf_load<T, VD>(rs, hl_off, hv);
+            for (int i = 0; i < RW; ++i) buf_load<Word, NV>(rs, ld_off[i], v[i]);
+            buf_load<Word, NV>(rs, hl_off, hv);
         };
-        auto decode = [&](const T (&w)[RW][VD], float (&f)[RW][VD]) {
+        auto decode = [&](const Word (&w)[RW][NV], float (&f)[RW][VD]) {
 #pragma unroll
             for (int i = 0; i < RW; ++i)
 #pragma unroll
-                for (int j = 0; j < VD; ++j) f[i][j] = Tr::dec(w[i][j], off);
+                for (int j = 0; j < VD; ++j) f[i][j] = LW::cell(w[i], j, off);
         };
         // a plane's boundary rows and edge cells into LDS
         auto stage = [&](LV (*L)[NL], float (*E)[NWD][RW][2], const float (&v)[RW][VD]) {
@@ -292,8 +380,8 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         // coeff * sum (stochastically: with the word ra ^ B(sweep, plane), where
         // `key` is the sweep's key)
         auto sweep = [&](auto halo, int p, const float (&lo)[RW][VD], const float (&mid)[RW][VD],
-                         const float (&hi)[RW][VD], const T (&midw)[RW][VD], const LV (*L)[NL],
-                         const float (*E)[NWD][RW][2], uint32_t key, T (&out)[RW][VD]) {
+                         const float (&hi)[RW][VD], const Word (&midw)[RW][NV], const LV (*L)[NL],
+                         const float (*E)[NWD][RW][2], uint32_t key, Word (&out)[RW][NV]) {
             constexpr bool HALO = decltype(halo)::value;
             const bool pint = p >= r && p < H - r;
             const uint32_t rb = SR ? sr_plane_bits(key, s.plane0 + (uint32_t)p) : 0u;  // uniform
@@ -321,7 +409,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
                 const float eright = E[ww][wd < NWD - 1 ? wd + 1 : NWD - 1][i][0];
                 if (lane == 0) left = eleft;
                 if (lane == kWave - 1) right = eright;
-                const bool face_row = !pint || !row_int[i];
+                float val[VD];
 #pragma unroll
                 for (int j = 0; j < VD; ++j) {
                     const float dm = j > 0 ? mid[i][j - 1] : left;
@@ -331,22 +419,29 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
                     acc = acc + wp[j];
                     acc = acc + dm;
                     acc = acc + dp;
-                    const bool face = face_row || (!dint && (d + j < r || d + j >= D - r));
-                    T v;
-                    if constexpr (SR) {
-                        v = Tr::enc_sr(coeff * acc, off, ra[i][j] ^ rb);
-                    } else {
-                        v = Tr::enc(coeff * acc, off);
-                    }
-                    out[i][j] = face ? midw[i][j] : v;
+                    val[j] = coeff * acc;
                 }
+                Word nw[NV];
+                if constexpr (SR) {
+                    uint32_t rw[VD];
+#pragma unroll
+                    for (int j = 0; j < VD; ++j) rw[j] = ra[i][j] ^ rb;
+                    LW::put_sr(nw, val, off, rw);
+                } else {
+                    LW::put(nw, val, off);
+                }
+                // faces keep their word: the whole row on a face plane or row
+                // (uniform), else the lane's d-face cells
+                const uint32_t frow = (!pint || !row_int[i]) ? ~0u : 0u;
+#pragma unroll
+                for (int k = 0; k < NV; ++k) out[i][k] = LW::merge(frow | dmask[k], midw[i][k], nw[k]);
             }
         };
 
 #pragma unroll
         for (int i = 0; i < RW; ++i)
 #pragma unroll
-            for (int j = 0; j < VD; ++j) s1[0][i][j] = s1[1][i][j] = T(0);
+            for (int k = 0; k < NV; ++k) s1[0][i][k] = s1[1][i][k] = Word(0);
         // prologue: each load followed by discarded stores, the order of the
         // steady state, so the wait counts the compiler derives at the loop head
         // hold on every path into it
@@ -356,7 +451,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
             for (int k = 0; k < PF + 2; ++k) {
                 load(ha - 1 + k, in[k], hal[k]);
 #pragma unroll
-                for (int i = 0; i < RW; ++i) buf_store<T, VD>(none, kNoAccess, s1[0][i]);
+                for (int i = 0; i < RW; ++i) buf_store<Word, NV>(none, kNoAccess, s1[0][i]);
             }
         }
 
@@ -375,7 +470,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
             if (top || bottom) {
                 float fh[VD];
 #pragma unroll
-                for (int j = 0; j < VD; ++j) fh[j] = Tr::dec(hal[I1][j], off);
+                for (int j = 0; j < VD; ++j) fh[j] = LW::cell(hal[I1], j, off);
                 lds_put<VD>(&lin[b][hslot][col], fh);
             }
             stage(ls1[b], les1[b], gm);
@@ -389,10 +484,10 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
                 for (int i = 0; i < RW; ++i)
 #pragma unroll
-                    for (int j = 0; j < VD; ++j) s1[I2][i][j] = T(0);
+                    for (int k = 0; k < NV; ++k) s1[I2][i][k] = Word(0);
             }
             const bool out2 = p2 >= h0 && p2 < h1;  // uniform across the block
-            T o[RW][VD];
+            Word o[RW][NV];
             if (out2) {
                 float glo[RW][VD], ghi[RW][VD];
                 decode(s1[I0], glo);
@@ -405,7 +500,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
                         for (int j = 0; j < VD; ++j) {
                             if (d + j < D) {
-                                const double dv = (double)Tr::dec_raw(o[i][j]) - (double)Tr::dec_raw(s1[I1][i][j]);
+                                const double dv = (double)LW::raw(o[i], j) - (double)LW::raw(s1[I1][i], j);
                                 rr += dv * dv;
                             }
                         }
@@ -415,11 +510,11 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
                 for (int i = 0; i < RW; ++i)
 #pragma unroll
-                    for (int j = 0; j < VD; ++j) o[i][j] = T(0);
+                    for (int k = 0; k < NV; ++k) o[i][k] = Word(0);
             }
             const auto rs = plane_rsrc((T*)s.dst + (size_t)(out2 ? p2 : 0) * plane, out2 ? plane_bytes : 0);
 #pragma unroll
-            for (int i = 0; i < RW; ++i) buf_store<T, VD>(rs, st_off[i], o[i]);
+            for (int i = 0; i < RW; ++i) buf_store<Word, NV>(rs, st_off[i], o[i]);
             b ^= 1;
         };
         // whole rounds of NQ steps: a step past hend loads and stores through
