@@ -302,7 +302,7 @@ typedef struct pskel_tune_query {
 } pskel_tune_query_t;
 
 #define PSKEL_TUNER_MAX_CANDIDATES 256
-#define PSKEL_TUNER_NFEAT 24
+#define PSKEL_TUNER_NFEAT 26
 
 typedef struct pskel_mlp pskel_mlp_t;
 

@@ -143,7 +143,7 @@ class FieldDesc(C.Structure):
 
 
 TUNER_MAX_CANDIDATES = 256
-TUNER_NFEAT = 24
+TUNER_NFEAT = 26
 
 STRUCTS = [StencilDesc, LaunchCfg, RunStats, GridDesc, GridInfo, FieldDesc]
 

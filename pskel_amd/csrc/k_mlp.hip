@@ -224,6 +224,7 @@ hipError_t backward(pskel_mlp_t* m, int n, const float* x_std)
 
 namespace pskel {
 pskel_status_t mlp_predict_host(const pskel_mlp_t* mc, const float* x, size_t n, float* y);
+int mlp_inputs(const pskel_mlp_t* m) { return m ? m->sizes[0] : 0; }
 }
 
 extern "C" {

@@ -19,6 +19,7 @@ using namespace pskel;
 
 namespace pskel {
 pskel_status_t mlp_predict_host(const pskel_mlp_t* m, const float* x, size_t n, float* y);
+int mlp_inputs(const pskel_mlp_t* m);
 }
 
 namespace {
@@ -41,7 +42,10 @@ void load_shipped_model()
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return;
     pskel_mlp_t* m = nullptr;
-    if (pskel_mlp_load(path.c_str(), dev, &m) == PSKEL_OK) g_default_model = m;
+    if (pskel_mlp_load(path.c_str(), dev, &m) != PSKEL_OK) return;
+    // a model trained on another feature set is not used (the heuristic is)
+    if (mlp_inputs(m) == PSKEL_TUNER_NFEAT) g_default_model = m;
+    else pskel_mlp_destroy(m);
 }
 
 pskel_launch_cfg_t tuned(int tx, int ty, int tz, int k)
@@ -99,11 +103,13 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
                     if (lds > 64 * 1024) continue;
                     v.push_back(tuned(vd, rw, hs, 1));
                 }
-        if (!narrow && q.iterations >= 2) {
+        if (q.iterations >= 2) {  // two sweeps per launch: the shapes built for each storage
             v.push_back(fused(4, 3, 768, 2));
             v.push_back(fused(4, 2, 1024, 2));
-            v.push_back(fused(4, 2, 512, 2));
-            v.push_back(fused(2, 4, 1024, 4));
+            if (!narrow) {
+                v.push_back(fused(4, 2, 512, 2));
+                v.push_back(fused(2, 4, 1024, 4));
+            }
         }
         v.push_back(plain(64, 4));
     } else {
@@ -114,6 +120,31 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
 }
 
 float lg2(double x) { return (float)std::log2(std::max(x, 1.0)); }
+
+// The fraction of a 3D candidate's tile lanes (along d) and rows (along w) that
+// hold grid cells: a tile much wider than the grid leaves lanes idle.  1 in 2D.
+float fill3d(const pskel_tune_query_t& q, const pskel_launch_cfg_t& c)
+{
+    if (q.ndim != 3) return 1.f;
+    const double D = (double)q.depth, W = (double)q.width;
+    auto fit = [](double n, double t) { return n / (std::ceil(n / t) * t); };
+    if (c.variant != PSKEL_VARIANT_TUNED)  // plain: block_x along d, block_y along w
+        return (float)(fit(D, std::max(c.block_x, 1)) * fit(W, std::max(c.block_y, 1)));
+    if (c.time_tile >= 2) {  // fused pair: extended tiles overlapping by a lane vector and two rows
+        const int vd = c.tile_x > 0 ? c.tile_x : 4, rw = c.tile_y > 0 ? c.tile_y : 3;
+        const int nw = c.block_x > 0 ? c.block_x / 64 : 12;
+        const int nwd = c.block_y > 0 ? c.block_y % 16 : 2;
+        if (nwd <= 0 || nw < nwd) return 1.f;
+        const double td = 64.0 * nwd * vd, tw = (double)(nw / nwd) * rw;
+        const double ntd = D <= td ? 1.0 : 1.0 + std::ceil((D - td) / (td - 2.0 * vd));
+        const double ntw = W <= tw ? 1.0 : 1.0 + std::ceil((W - tw) / (tw - 2.0));
+        return (float)(D / (ntd * td) * W / (ntw * tw));
+    }
+    // one sweep per launch: tiles of 64 lanes x VD cells by 4 waves x RW rows
+    const int vd = c.tile_x > 0 ? c.tile_x : (q.storage == PSKEL_DTYPE_F32 ? 1 : 2);
+    const int rw = c.tile_y > 0 ? c.tile_y : 4;
+    return (float)(fit(D, 64.0 * vd) * fit(W, 4.0 * rw));
+}
 
 void features(const pskel_tune_query_t& q, const pskel_stencil_desc_t& d, const pskel_launch_cfg_t& c, float* f)
 {
@@ -146,6 +177,9 @@ void features(const pskel_tune_query_t& q, const pskel_stencil_desc_t& d, const 
     f[i++] = (float)std::max(c.time_tile, 1) * (q.ndim == 2 ? 1.f : 0.f) / (float)std::max(c.tile_y, 1);
     f[i++] = lg2((double)q.width * q.height * (q.ndim == 3 ? q.depth : 1) /
                  (double)std::max(1, c.tile_z > 0 && q.ndim == 3 ? c.tile_z : 1));
+    f[i++] = fill3d(q, c);
+    // fused 16-bit pairs are bound by instruction issue, f32 ones by HBM
+    f[i++] = (q.ndim == 3 && c.time_tile >= 2 && q.storage != PSKEL_DTYPE_F32) ? 1.f : 0.f;
     f[i++] = 1.f;  // bias-like constant
 }
 
@@ -164,12 +198,20 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
         return tuned(0, 2, 4, 1);
     }
     if (q.ndim == 3 && pat == PAT_J7) {
-        if (q.storage == PSKEL_DTYPE_F32) {
-            const double cells = (double)q.width * q.height * q.depth;
-            if (q.iterations >= 2 && cells >= (1 << 21)) return fused(4, 3, 768, 2);
-            return tuned(4, 1, 128, 1);
-        }
-        return q.storage == PSKEL_DTYPE_BF16 ? tuned(2, 2, 32, 1) : tuned(4, 2, 32, 1);
+        const double cells = (double)q.width * q.height * q.depth;
+        const pskel_launch_cfg_t pair = fused(4, 3, 768, 2);
+        if (q.iterations >= 2 && cells >= (1 << 21) && fill3d(q, pair) >= 0.7f) return pair;
+        // one sweep per launch: the widest lane vector (up to the best at 512^3)
+        // whose 64-lane rows the depth fills
+        const bool f32 = q.storage == PSKEL_DTYPE_F32;
+        const int vmax = f32 ? 4 : (q.storage == PSKEL_DTYPE_BF16 ? 2 : 4);
+        int vd = f32 ? 1 : 2;
+        for (int v = vmax; v > vd; v /= 2)
+            if ((double)q.depth >= 0.75 * 64.0 * v) {
+                vd = v;
+                break;
+            }
+        return f32 ? tuned(vd, 1, 128, 1) : tuned(vd, 2, 32, 1);
     }
     return plain(64, 4);
 }
@@ -229,6 +271,8 @@ pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_que
         *out = heuristic(*q, *d);
         return PSKEL_OK;
     }
+    if (mlp_inputs(model) != PSKEL_TUNER_NFEAT)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "the model's input width is not the tuner's feature count");
     const auto v = candidates(*q, *d);
     std::vector<float> x(v.size() * PSKEL_TUNER_NFEAT), y(v.size());
     for (size_t i = 0; i < v.size(); ++i) features(*q, *d, v[i], &x[i * PSKEL_TUNER_NFEAT]);

@@ -35,15 +35,22 @@ def test_3d_candidates_fit_lds():
                 assert (c.tile_x in (1, 2, 4)) if st == "f32" else (c.tile_x in (2, 4, 8))
 
 
-def test_3d_fused_candidates_f32_only():
-    """Two fused sweeps per launch are offered for the f32 7-point Jacobi
-    with at least two sweeps, never for 16-bit storage."""
+def test_3d_fused_candidates_per_storage():
+    """Two fused sweeps per launch are offered for the 7-point Jacobi with at
+    least two sweeps: the shapes built for each storage (two for 16-bit, both
+    also built for f32)."""
     fused = lambda q: [c for c in T.candidates(q) if c.time_tile == 2]  # noqa: E731
-    assert len(fused(_q((64, 64, 64), "f32", "jacobi3d", 10))) >= 3
+    f32 = fused(_q((64, 64, 64), "f32", "jacobi3d", 10))
+    b16 = fused(_q((64, 64, 64), "bf16", "jacobi3d", 10))
+    assert len(f32) == 4 and len(b16) == 2
+    assert {(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in b16} <= \
+        {(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in f32}
     assert fused(_q((64, 64, 64), "f32", "jacobi3d", 1)) == []
-    assert fused(_q((64, 64, 64), "bf16", "jacobi3d", 10)) == []
-    h = T.heuristic_choice(_q((512, 512, 512), "f32", "jacobi3d", 100))
-    assert h.time_tile == 2
+    assert fused(_q((64, 64, 64), "f16", "jacobi3d", 1)) == []
+    for storage in ("f32", "bf16", "f16"):
+        h = T.heuristic_choice(_q((512, 512, 512), storage, "jacobi3d", 100))
+        assert (h.time_tile, h.tile_x, h.tile_y, h.block_x, h.block_y) == (2, 4, 3, 768, 2)
+        assert T.heuristic_choice(_q((512, 512, 512), storage, "jacobi3d", 1)).time_tile == 1
 
 
 def test_features_shape_and_determinism():
