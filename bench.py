@@ -334,9 +334,11 @@ def cpu_baseline(name, seconds_target=20.0):
     s = stencil_of(name)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or oracle.num_procs()
     threads = min(threads, 16)
-    # calibrate with 2 sweeps, then time the sample
+    # calibrate with 2 sweeps (the second run: the first pays for first-touch
+    # page faults), then time the sample
     run = (lambda n: oracle.run_3d(s, f, n, threads=threads)) if len(w["shape"]) == 3 else \
         (lambda n: oracle.run_2d(s, f, n, threads=threads))
+    run(2)
     run(2)
     per = oracle.last_seconds() / 2
     n = max(2, min(w["sweeps"], int(seconds_target / max(per, 1e-6))))

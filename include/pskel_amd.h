@@ -318,6 +318,10 @@ pskel_status_t pskel_tuner_features(const pskel_tune_query_t* q, const pskel_ste
 pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_query_t* q,
                                   const pskel_stencil_desc_t* d, pskel_launch_cfg_t* out);
 pskel_status_t pskel_tuner_set_default_model(pskel_mlp_t* model);
+/* The fixed per-family heuristic alone: what pskel_tuner_choose falls back to
+ * without a model (for measuring the model against it). */
+pskel_status_t pskel_tuner_heuristic(const pskel_tune_query_t* q, const pskel_stencil_desc_t* d,
+                                     pskel_launch_cfg_t* out);
 /* Device time per sweep of `sweeps` sweeps of candidate c on grid g (median of
  * reps runs after one warm-up; the grid state advances). */
 pskel_status_t pskel_tuner_measure(pskel_grid_t* g, const pskel_stencil_desc_t* d, size_t sweeps,

@@ -201,6 +201,7 @@ def _declare(lib):
          C.POINTER(SZ))
     _sig(lib, "pskel_tuner_features", st, C.POINTER(TuneQuery), C.POINTER(StencilDesc), C.POINTER(LaunchCfg), F)
     _sig(lib, "pskel_tuner_choose", st, P, C.POINTER(TuneQuery), C.POINTER(StencilDesc), C.POINTER(LaunchCfg))
+    _sig(lib, "pskel_tuner_heuristic", st, C.POINTER(TuneQuery), C.POINTER(StencilDesc), C.POINTER(LaunchCfg))
     _sig(lib, "pskel_tuner_set_default_model", st, P)
     _sig(lib, "pskel_tuner_measure", st, P, C.POINTER(StencilDesc), SZ, C.POINTER(LaunchCfg), I32, C.POINTER(D))
     _sig(lib, "pskel_mlp_create", st, C.POINTER(I32), I32, C.c_uint64, I32, C.POINTER(P))

@@ -250,6 +250,16 @@ pskel_status_t pskel_tuner_features(const pskel_tune_query_t* q, const pskel_ste
     return PSKEL_OK;
 }
 
+pskel_status_t pskel_tuner_heuristic(const pskel_tune_query_t* q, const pskel_stencil_desc_t* d,
+                                     pskel_launch_cfg_t* out)
+{
+    pskel_status_t st = check_query(q, d);
+    if (st != PSKEL_OK) return st;
+    if (!out) return fail(PSKEL_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = heuristic(*q, *d);
+    return PSKEL_OK;
+}
+
 pskel_status_t pskel_tuner_set_default_model(pskel_mlp_t* model)
 {
     std::call_once(g_model_once, [] {});  // an explicit choice wins over the shipped file

@@ -269,7 +269,7 @@ class CostModel:
 def heuristic_choice(q: Query) -> L.LaunchCfg:
     cq, d = q.c_query(), q.stencil().desc()
     out = L.LaunchCfg()
-    L.check(L.lib().pskel_tuner_choose(None, C.byref(cq), C.byref(d), C.byref(out)))
+    L.check(L.lib().pskel_tuner_heuristic(C.byref(cq), C.byref(d), C.byref(out)))
     return out
 
 
