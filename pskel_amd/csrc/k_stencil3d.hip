@@ -13,6 +13,8 @@
 //              planes keeping h-1/h/h+1 in registers and staging plane h (with
 //              its one-cell w/d halo) in a double-buffered LDS tile, one barrier
 //              per plane.  Vector loads of VD cells per lane along d.
+//   k3d_march2 two fused sweeps per launch of the 7-point scaled sum, defined in
+//              k_march3d_t2.hpp and dispatched from launch_3d_ttile below.
 //
 // The residual ||u_new - u_old||_2 is accumulated in FP64 per block (fixed
 // reduction order) and reduced by a second single-block pass; FP64 is used for

@@ -188,30 +188,37 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
 {
     const int pat = classify(d, q.ndim);
     if (q.ndim == 2 && pat == PAT_CLOUD) return tuned(0, 2, 4, 1);
-    if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9 || pat == PAT_LIFE8)) {
+    if (q.ndim == 2 && pat == PAT_LIFE8) {
         const double cells = (double)q.width * q.height;
         // bit-packed tiles are 1984 columns wide: narrower grids leave lanes idle
-        if (pat == PAT_LIFE8 && q.iterations >= 8 && cells >= (1 << 20) && q.width >= 2048)
-            return tuned(0, 32, 0, 8);
-        if (pat == PAT_LIFE8) return tuned(0, 2, 4, 1);
-        if (q.iterations >= 4 && cells >= (1 << 20)) return tuned(0, 16, 0, pat == PAT_BOX9 ? 3 : 4);
+        if (q.iterations >= 8 && cells >= (1 << 20) && q.width >= 2048) return tuned(0, 32, 0, 8);
+        // small grids: one cell per lane keeps enough blocks in flight
+        if (cells < (1 << 18)) return plain(64, 4);
+        return tuned(0, 1, 4, 1);
+    }
+    if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9)) {
+        // temporal tiles of 16 rows per lane pay at every grid size measured (64 .. 4096 a side)
+        if (q.iterations >= 2) return tuned(0, 16, 0, (int)std::min<size_t>(q.iterations, 4));
         return tuned(0, 2, 4, 1);
     }
     if (q.ndim == 3 && pat == PAT_J7) {
-        const double cells = (double)q.width * q.height * q.depth;
+        // the fused pair won on every grid measured that fills a third of its
+        // tiles (down to 199 deep)
         const pskel_launch_cfg_t pair = fused(4, 3, 768, 2);
-        if (q.iterations >= 2 && cells >= (1 << 21) && fill3d(q, pair) >= 0.7f) return pair;
-        // one sweep per launch: the widest lane vector (up to the best at 512^3)
-        // whose 64-lane rows the depth fills
+        if (q.iterations >= 2 && fill3d(q, pair) >= 0.3f) return pair;
+        // one sweep per launch: a 64-lane row as wide as the depth needs, and
+        // segments short enough for about 1024 blocks
         const bool f32 = q.storage == PSKEL_DTYPE_F32;
-        const int vmax = f32 ? 4 : (q.storage == PSKEL_DTYPE_BF16 ? 2 : 4);
-        int vd = f32 ? 1 : 2;
-        for (int v = vmax; v > vd; v /= 2)
-            if ((double)q.depth >= 0.75 * 64.0 * v) {
-                vd = v;
+        const int vd = f32 ? (q.depth <= 64 ? 1 : q.depth <= 128 ? 2 : 4) : (q.depth <= 128 ? 2 : 4);
+        const int rw = f32 ? 1 : 2;
+        const double tiles = std::ceil((double)q.depth / (64.0 * vd)) * std::ceil((double)q.width / (4.0 * rw));
+        int hs = 16;
+        for (int h : {128, 64, 32})
+            if (tiles * std::ceil((double)q.height / h) >= 1024) {
+                hs = h;
                 break;
             }
-        return f32 ? tuned(vd, 1, 128, 1) : tuned(vd, 2, 32, 1);
+        return tuned(vd, rw, hs, 1);
     }
     return plain(64, 4);
 }

@@ -81,7 +81,8 @@ def main(argv=None):
         "model_speedup_over_plain_geomean": float(np.exp(np.mean(np.log(gain)))),
         "per_query": [{"shape": list(x["q"].shape), "stencil": x["q"].stencil_name, "storage": x["q"].storage,
                        "iterations": x["q"].iterations, "best_ms": x["best_ms"], "model_ms": x["model_ms"],
-                       "plain_ms": x["plain_ms"], "regret": x["regret"]} for x in reg],
+                       "heuristic_ms": x["heuristic_ms"], "plain_ms": x["plain_ms"], "regret": x["regret"],
+                       "heuristic_regret": x["heuristic_regret"]} for x in reg],
         "seconds": time.time() - t0,
     }
     log(json.dumps({k: v for k, v in report.items() if k != "per_query"}))

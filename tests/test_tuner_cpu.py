@@ -53,6 +53,34 @@ def test_3d_fused_candidates_per_storage():
         assert T.heuristic_choice(_q((512, 512, 512), storage, "jacobi3d", 1)).time_tile == 1
 
 
+def test_3d_heuristic_fits_the_depth():
+    """The heuristic takes the 512-cell fused pair from 30% fill of its
+    tiles up; below, one sweep per launch with a 64-lane row as wide as the
+    depth needs and segments short enough for about 1024 blocks."""
+    h = lambda shape, st: T.heuristic_choice(_q(shape, st, "jacobi3d", 100))  # noqa: E731
+    c = h((269, 487, 42), "f32")
+    assert (c.time_tile, c.tile_x, c.tile_y, c.tile_z) == (1, 1, 1, 32)  # 122 tiles x 9 segments
+    assert h((211, 243, 66), "f32").tile_x == 2 and h((211, 243, 66), "f32").time_tile == 1
+    assert h((211, 243, 100), "f32").tile_x == 2
+    for shape, st in (((334, 359, 199), "f32"), ((429, 281, 449), "f16"), ((48, 85, 242), "f16"),
+                      ((497, 48, 244), "bf16")):
+        assert h(shape, st).time_tile == 2, shape
+    assert h((64, 64, 150), "bf16").time_tile == 1 and h((64, 64, 150), "bf16").tile_x == 4
+    assert T.heuristic_choice(_q((429, 281, 449), "f16", "jacobi3d", 1)).time_tile == 1
+
+
+def test_2d_heuristic():
+    """Temporal tiles of 16 rows for the Jacobi and 9-point stencils at every
+    size (depth 4, or the sweep count when smaller); life: the plain kernel on
+    small grids, bit tiles on wide ones."""
+    c = T.heuristic_choice(_q((125, 205), "f32", "jacobi2d", 158))
+    assert (c.tile_y, c.time_tile) == (16, 4)
+    assert T.heuristic_choice(_q((2903, 1983), "f32", "diffusion9", 3)).time_tile == 3
+    assert T.heuristic_choice(_q((229, 71), "u8", "life", 107)).variant == 0
+    assert T.heuristic_choice(_q((8194, 8194), "u8", "life", 500)).time_tile == 8
+    assert T.heuristic_choice(_q((1683, 1653), "u8", "life", 29)).time_tile == 1
+
+
 def test_features_shape_and_determinism():
     q = _q((300, 700), "f32", "jacobi2d", 50)
     c = T.candidates(q)
