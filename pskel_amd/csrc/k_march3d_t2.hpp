@@ -86,10 +86,13 @@ inline bool march2_supported(const March2Shape& m, int storage)
         return ((((vd * 16 + rw) * 16 + nwd) * 16) + nww) * 16 + pf;
     };
     const int k = key(m.vd, m.rw, m.nwd, m.nww, m.pf);
-    if (storage != PSKEL_DTYPE_F32) return k == key(4, 3, 2, 6, 1) || k == key(4, 2, 2, 8, 1);
+    if (storage != PSKEL_DTYPE_F32)
+        return k == key(4, 3, 2, 6, 1) || k == key(4, 2, 2, 8, 1) || k == key(4, 3, 1, 12, 1) ||
+               k == key(2, 3, 1, 12, 1);
     return k == key(2, 4, 4, 4, 1) || k == key(2, 4, 4, 2, 1) || k == key(4, 2, 2, 8, 1) ||
            k == key(4, 2, 2, 4, 1) || k == key(2, 4, 4, 4, 2) || k == key(2, 4, 2, 4, 1) ||
-           k == key(4, 4, 2, 4, 1) || k == key(4, 3, 2, 6, 1);
+           k == key(4, 4, 2, 4, 1) || k == key(4, 3, 2, 6, 1) || k == key(4, 3, 1, 12, 1) ||
+           k == key(2, 3, 1, 12, 1);
 }
 
 hipError_t march3d_t2_f32(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st);
@@ -627,10 +630,14 @@ hipError_t march2_launch(const Sweep3D& s, const Points& p, const March2Shape& m
     X(2, 4, 4, 4, 2)               \
     X(2, 4, 2, 4, 1)               \
     X(4, 4, 2, 4, 1)               \
-    X(4, 3, 2, 6, 1)
+    X(4, 3, 2, 6, 1)               \
+    X(4, 3, 1, 12, 1)              \
+    X(2, 3, 1, 12, 1)
 #define PSKEL_MARCH2_SHAPES_16(X) \
     X(4, 3, 2, 6, 1)              \
-    X(4, 2, 2, 8, 1)
+    X(4, 2, 2, 8, 1)              \
+    X(4, 3, 1, 12, 1)             \
+    X(2, 3, 1, 12, 1)
 
 template <int ST>
 size_t march2_blocks(const Sweep3D& s, const March2Shape& m)

@@ -106,6 +106,8 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
         if (q.iterations >= 2) {  // two sweeps per launch: the shapes built for each storage
             v.push_back(fused(4, 3, 768, 2));
             v.push_back(fused(4, 2, 1024, 2));
+            v.push_back(fused(4, 3, 768, 1));  // 256-cell rows, 36-row tiles
+            v.push_back(fused(2, 3, 768, 1));  // 128-cell rows
             if (!narrow) {
                 v.push_back(fused(4, 2, 512, 2));
                 v.push_back(fused(2, 4, 1024, 4));
@@ -202,10 +204,20 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
         return tuned(0, 2, 4, 1);
     }
     if (q.ndim == 3 && pat == PAT_J7) {
-        // the fused pair won on every grid measured that fills a third of its
-        // tiles (down to 199 deep)
-        const pskel_launch_cfg_t pair = fused(4, 3, 768, 2);
-        if (q.iterations >= 2 && fill3d(q, pair) >= 0.3f) return pair;
+        // the fused pair whose tiles the grid fills best (512-, 256- or
+        // 128-cell rows; the widest on ties); it won on every grid measured,
+        // down to 42 deep (29% of a 128-cell row)
+        const pskel_launch_cfg_t pairs[] = {fused(4, 3, 768, 2), fused(4, 3, 768, 1), fused(2, 3, 768, 1)};
+        int bi = 0;
+        float bf = fill3d(q, pairs[0]);
+        for (int i = 1; i < 3; ++i) {
+            const float f = fill3d(q, pairs[i]);
+            if (f > bf + 0.05f) {
+                bf = f;
+                bi = i;
+            }
+        }
+        if (q.iterations >= 2 && bf >= 0.2f) return pairs[bi];
         // one sweep per launch: a 64-lane row as wide as the depth needs, and
         // segments short enough for about 1024 blocks
         const bool f32 = q.storage == PSKEL_DTYPE_F32;

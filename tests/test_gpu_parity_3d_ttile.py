@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 # (cells per lane, rows per wave, waves along d, waves along w, planes prefetched):
 # the instantiated fused-tile shapes
 SHAPES_T2 = [(2, 4, 4, 4, 1), (2, 4, 4, 2, 1), (4, 2, 2, 8, 1), (4, 2, 2, 4, 1), (2, 4, 4, 4, 2), (2, 4, 2, 4, 1),
-             (4, 4, 2, 4, 1), (4, 3, 2, 6, 1)]
+             (4, 4, 2, 4, 1), (4, 3, 2, 6, 1), (4, 3, 1, 12, 1), (2, 3, 1, 12, 1)]
 
 # grids: tiny, one tile, ragged, many w tiles, several d tiles (D > 64 * 8)
 GRIDS = [(1, 1, 1), (2, 3, 4), (3, 3, 3), (5, 9, 70), (17, 9, 70), (33, 40, 65), (20, 70, 300), (9, 23, 1100)]
@@ -168,7 +168,7 @@ def test_fused_depth_is_a_hint(gpu):
     assert np.array_equal(got, want16)
 
 
-SHAPES_16 = [(4, 3, 2, 6, 1), (4, 2, 2, 8, 1)]
+SHAPES_16 = [(4, 3, 2, 6, 1), (4, 2, 2, 8, 1), (4, 3, 1, 12, 1), (2, 3, 1, 12, 1)]
 
 
 def _run16(words, iters, store, tshape, per=0, offset=0.5, rounding="nearest", seed=0, **kw):

@@ -37,12 +37,12 @@ def test_3d_candidates_fit_lds():
 
 def test_3d_fused_candidates_per_storage():
     """Two fused sweeps per launch are offered for the 7-point Jacobi with at
-    least two sweeps: the shapes built for each storage (two for 16-bit, both
+    least two sweeps: the shapes built for each storage (four for 16-bit, all
     also built for f32)."""
     fused = lambda q: [c for c in T.candidates(q) if c.time_tile == 2]  # noqa: E731
     f32 = fused(_q((64, 64, 64), "f32", "jacobi3d", 10))
     b16 = fused(_q((64, 64, 64), "bf16", "jacobi3d", 10))
-    assert len(f32) == 4 and len(b16) == 2
+    assert len(f32) == 6 and len(b16) == 4
     assert {(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in b16} <= \
         {(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in f32}
     assert fused(_q((64, 64, 64), "f32", "jacobi3d", 1)) == []
@@ -54,18 +54,19 @@ def test_3d_fused_candidates_per_storage():
 
 
 def test_3d_heuristic_fits_the_depth():
-    """The heuristic takes the 512-cell fused pair from 30% fill of its
-    tiles up; below, one sweep per launch with a 64-lane row as wide as the
-    depth needs and segments short enough for about 1024 blocks."""
+    """The heuristic takes the fused pair whose 512-, 256- or 128-cell rows
+    the grid fills best, from 20% fill of its tiles up; below, one sweep per
+    launch with a 64-lane row as wide as the depth needs and segments short
+    enough for about 1024 blocks."""
     h = lambda shape, st: T.heuristic_choice(_q(shape, st, "jacobi3d", 100))  # noqa: E731
-    c = h((269, 487, 42), "f32")
+    key = lambda c: (c.time_tile, c.tile_x, c.tile_y, c.block_x, c.block_y)  # noqa: E731
+    assert key(h((512, 512, 512), "f32")) == (2, 4, 3, 768, 2)
+    assert key(h((401, 258, 252), "f16")) == (2, 4, 3, 768, 1)
+    assert key(h((334, 359, 199), "f32")) == (2, 4, 3, 768, 1)
+    assert key(h((211, 243, 66), "f32")) == (2, 2, 3, 768, 1)
+    assert key(h((269, 487, 42), "f32")) == (2, 2, 3, 768, 1)
+    c = h((269, 487, 24), "f32")  # 17% of a 128-cell row
     assert (c.time_tile, c.tile_x, c.tile_y, c.tile_z) == (1, 1, 1, 32)  # 122 tiles x 9 segments
-    assert h((211, 243, 66), "f32").tile_x == 2 and h((211, 243, 66), "f32").time_tile == 1
-    assert h((211, 243, 100), "f32").tile_x == 2
-    for shape, st in (((334, 359, 199), "f32"), ((429, 281, 449), "f16"), ((48, 85, 242), "f16"),
-                      ((497, 48, 244), "bf16")):
-        assert h(shape, st).time_tile == 2, shape
-    assert h((64, 64, 150), "bf16").time_tile == 1 and h((64, 64, 150), "bf16").tile_x == 4
     assert T.heuristic_choice(_q((429, 281, 449), "f16", "jacobi3d", 1)).time_tile == 1
 
 
