@@ -79,7 +79,8 @@ inline March2Shape march2_shape(const Sweep3D& s, const pskel_launch_cfg_t& cfg)
     return m;
 }
 
-// the instantiated shapes (vd, rw, nwd, nww, pf): several for f32, two for 16-bit storage
+// the instantiated shapes (vd, rw, nwd, nww, pf): ten for f32, four for 16-bit
+// storage (rows of 512, 256 and 128 cells along d)
 inline bool march2_supported(const March2Shape& m, int storage)
 {
     auto key = [](int vd, int rw, int nwd, int nww, int pf) {

@@ -277,13 +277,16 @@ def heuristic_choice(q: Query) -> L.LaunchCfg:
 # data collection
 # ---------------------------------------------------------------------------
 def random_queries(n: int, seed: int, kinds=("jacobi2d", "diffusion9", "life", "jacobi3d")) -> List[Query]:
-    """Random shapes: 2D sides log-uniform in [64, 4096]; 3D sides log-uniform
-    in [32, 512] for half the 3D queries and in [192, 640] for the other half,
-    where the HBM-bound kernels part ways (at most 2^27 cells); iterations
-    log-uniform in [10, 1000]."""
+    """Random shapes: 2D sides log-uniform in [64, 4096] for half the queries
+    of each 2D kind and in [1024, 8192] for the other half (at most 2^26
+    cells), where bit-packed life tiles and deep temporal tiles pay; 3D sides
+    log-uniform in [32, 512] for half the 3D queries and in [192, 640] for the
+    other half, where the HBM-bound kernels part ways (at most 2^27 cells);
+    iterations log-uniform in [10, 1000]."""
     rng = random.Random(seed)
     out = []
     n3 = 0
+    n2 = {}
     for i in range(n):
         kind = kinds[i % len(kinds)]
         it = int(round(math.exp(rng.uniform(math.log(10), math.log(1000)))))
@@ -297,7 +300,13 @@ def random_queries(n: int, seed: int, kinds=("jacobi2d", "diffusion9", "life", "
             st = rng.choice(["f32", "bf16", "f16"])
             out.append(Query(s, st, kind, it))
         else:
-            s = tuple(int(round(math.exp(rng.uniform(math.log(64), math.log(4096))))) for _ in range(2))
+            k2 = n2.get(kind, 0)
+            n2[kind] = k2 + 1
+            lo, hi = (64, 4096) if k2 % 2 == 0 else (1024, 8192)
+            while True:
+                s = tuple(int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(2))
+                if np.prod(s) <= 2 ** 26:
+                    break
             out.append(Query(s, "u8" if kind == "life" else "f32", kind, it))
     return out
 
