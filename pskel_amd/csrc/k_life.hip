@@ -8,7 +8,7 @@
 //   k_life_swar   16 cells per lane packed in two u64 words; horizontal
 //                 neighbours by byte shifts plus one cross-lane exchange per row,
 //                 counts and rule evaluated bytewise (SWAR), RY rows per lane.
-//   k_life_bits   K generations per launch (k_life_ttile's successor) on
+//   k_life_bits   K generations per launch on
 //                 cells packed one bit each: each wave evolves its own tile
 //                 in registers.
 #include "k_tiles2d.hpp"

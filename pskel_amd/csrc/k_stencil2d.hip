@@ -136,6 +136,13 @@ __global__ __launch_bounds__(256) void k2d_ttile(Sweep2D s, Points p)
     }
     __syncthreads();
 
+    // Level j: each thread walks one column x of the level's region down one
+    // of NS strips of rows, keeping the row above and its own row (x-1, x, x+1)
+    // in registers, so a cell costs three LDS reads (the row below).  Sums keep
+    // the mask order.
+    constexpr int NS = NT / 128;  // strips per column
+    static_assert(LX <= 128, "one thread per column of the loaded region");
+    const int col = tid % 128, strip = tid / 128;
 #pragma unroll
     for (int j = 1; j <= K; ++j) {
         const float* __restrict__ P = (j & 1) ? A : B;
@@ -143,29 +150,40 @@ __global__ __launch_bounds__(256) void k2d_ttile(Sweep2D s, Points p)
         const int xlo = 4 - (K - j), xhi = 4 + CX + (K - j);
         const int ylo = j, yhi = LY - j;
         const int nx = xhi - xlo, ny = yhi - ylo;
-        for (int q = tid; q < nx * ny; q += NT) {
-            const int y = ylo + q / nx, x = xlo + q % nx;
-            const int gh = gy0 + y, gw = gx0 + x;
-            if (gh < r || gh >= H - r || gw < r || gw >= Wd - r) continue;
-            const float* c = P + y * LX + x;
-            float acc;
-            if constexpr (PAT == PAT_J5) {
-                acc = c[-LX] + c[LX];
-                acc = acc + c[-1];
-                acc = acc + c[1];
-                acc = p.coeff * acc;
-            } else {
-                acc = p.wt[0] * c[-LX - 1];
-                acc = acc + p.wt[1] * c[-LX];
-                acc = acc + p.wt[2] * c[-LX + 1];
-                acc = acc + p.wt[3] * c[-1];
-                acc = acc + p.wt[4] * c[0];
-                acc = acc + p.wt[5] * c[1];
-                acc = acc + p.wt[6] * c[LX - 1];
-                acc = acc + p.wt[7] * c[LX];
-                acc = acc + p.wt[8] * c[LX + 1];
+        const int rows = (ny + NS - 1) / NS;
+        if (col < nx) {
+            const int x = xlo + col;
+            const int gw = gx0 + x;
+            const bool col_int = gw >= r && gw < Wd - r;
+            const int ya = ylo + strip * rows, yb = min(ya + rows, yhi);
+            const float* c = P + (ya - 1) * LX + x;
+            float um = c[-1], u0 = c[0], up = c[1];                 // row y - 1
+            float cm = c[LX - 1], c0 = c[LX], cp = c[LX + 1];       // row y
+            for (int y = ya; y < yb; ++y) {
+                c += LX;
+                const float dm = c[LX - 1], d0 = c[LX], dp = c[LX + 1];  // row y + 1
+                float acc;
+                if constexpr (PAT == PAT_J5) {
+                    acc = u0 + d0;
+                    acc = acc + cm;
+                    acc = acc + cp;
+                    acc = p.coeff * acc;
+                } else {
+                    acc = p.wt[0] * um;
+                    acc = acc + p.wt[1] * u0;
+                    acc = acc + p.wt[2] * up;
+                    acc = acc + p.wt[3] * cm;
+                    acc = acc + p.wt[4] * c0;
+                    acc = acc + p.wt[5] * cp;
+                    acc = acc + p.wt[6] * dm;
+                    acc = acc + p.wt[7] * d0;
+                    acc = acc + p.wt[8] * dp;
+                }
+                const int gh = gy0 + y;
+                if (col_int && gh >= r && gh < H - r) Q[y * LX + x] = acc;
+                um = cm; u0 = c0; up = cp;
+                cm = dm; c0 = d0; cp = dp;
             }
-            Q[y * LX + x] = acc;
         }
         __syncthreads();
     }
