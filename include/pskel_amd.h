@@ -128,9 +128,10 @@ typedef struct pskel_grid pskel_grid_t;
 
 /* Rounding of BF16/F16 stores.  STOCHASTIC rounds the magnitude up with
  * probability equal to the dropped fraction of an ulp, from random bits that
- * are a hash of (seed, index of the sweep since the last INPUT upload, unpitched
- * cell index (h*W + w)*D + d): unbiased, so slow relaxation does not stall below
- * half an ulp, and reproducible run to run. */
+ * are the XOR of a hash of (seed, in-plane cell index w*D + d) and a hash of
+ * (seed, index of the sweep since the last INPUT upload, plane h): unbiased, so
+ * slow relaxation does not stall below half an ulp, pairwise independent between
+ * distinct (cell, sweep) pairs, and reproducible run to run. */
 typedef enum pskel_rounding {
     PSKEL_ROUND_NEAREST = 0,
     PSKEL_ROUND_STOCHASTIC = 1
