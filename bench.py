@@ -62,7 +62,7 @@ TUNED = {
     "jacobi3d_bf16_single": dict(tile=(2, 2, 32)),
     # 8 generations per launch on bit-packed tiles of 32 rows per lane
     "life": dict(tile=(0, 32, 0), time_tile=8),
-    "jacobi2d": dict(tile=(0, 16, 0), time_tile=4),
+    "jacobi2d": dict(tile=(0, 8, 0), time_tile=4),
     "diffusion9": dict(tile=(0, 16, 0), time_tile=4),
     "cloudsim": dict(tile=(0, 1, 2)),
 }

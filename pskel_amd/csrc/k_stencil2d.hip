@@ -159,6 +159,7 @@ __global__ __launch_bounds__(256) void k2d_ttile(Sweep2D s, Points p)
             const float* c = P + (ya - 1) * LX + x;
             float um = c[-1], u0 = c[0], up = c[1];                 // row y - 1
             float cm = c[LX - 1], c0 = c[LX], cp = c[LX + 1];       // row y
+#pragma unroll 2
             for (int y = ya; y < yb; ++y) {
                 c += LX;
                 const float dm = c[LX - 1], d0 = c[LX], dp = c[LX + 1];  // row y + 1
@@ -204,6 +205,11 @@ hipError_t ttile_launch_k(const Sweep2D& s, const Points& p, int cy, hipStream_t
     constexpr int CX = 120;
     const dim3 block(256);
     switch (cy) {
+    case 8: {
+        const dim3 grid((s.W + CX - 1) / CX, (s.H + 7) / 8);
+        hipLaunchKernelGGL((k2d_ttile<PAT, K, CX, 8>), grid, block, 0, st, s, p);
+        break;
+    }
     case 16: {
         const dim3 grid((s.W + CX - 1) / CX, (s.H + 15) / 16);
         hipLaunchKernelGGL((k2d_ttile<PAT, K, CX, 16>), grid, block, 0, st, s, p);

@@ -65,7 +65,7 @@ def test_tuned_tile_shapes(gpu, tile):
 
 @pytest.mark.parametrize("stencil_name", ["jacobi2d", "diffusion9"])
 @pytest.mark.parametrize("k", [2, 3, 4])
-@pytest.mark.parametrize("cy", [16, 32, 64])
+@pytest.mark.parametrize("cy", [8, 16, 32, 64])
 def test_temporal_tiles(gpu, stencil_name, k, cy):
     s = getattr(K, stencil_name)()
     a = inputs.blobs_field(400, seed=7, nblobs=6, sigma=10.0)[:333, :397].copy()
