@@ -90,7 +90,7 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
                 for (int rows : {32, 40, 48}) v.push_back(tuned(0, rows, 0, k));
         } else if (pat != PAT_CLOUD) {  // field stencils run one sweep per launch
             for (int k = 2; k <= 4; ++k)
-                for (int cy : {8, 16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
+                for (int cy : {16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
         }
         v.push_back(plain(64, 4));
     } else if (q.ndim == 3 && pat == PAT_J7) {
@@ -199,10 +199,8 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
         return tuned(0, 1, 4, 1);
     }
     if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9)) {
-        // temporal tiles pay at every grid size measured (64 .. 8192 a side):
-        // 8 rows for the 5-point Jacobi, 16 for the 9-point stencil
-        if (q.iterations >= 2)
-            return tuned(0, pat == PAT_J5 ? 8 : 16, 0, (int)std::min<size_t>(q.iterations, 4));
+        // temporal tiles of 16 rows pay at every grid size measured (64 .. 8192 a side)
+        if (q.iterations >= 2) return tuned(0, 16, 0, (int)std::min<size_t>(q.iterations, 4));
         return tuned(0, 2, 4, 1);
     }
     if (q.ndim == 3 && pat == PAT_J7) {

@@ -71,11 +71,11 @@ def test_3d_heuristic_fits_the_depth():
 
 
 def test_2d_heuristic():
-    """Temporal tiles at every size (depth 4, or the sweep count when
-    smaller), 8 rows for the Jacobi and 16 for the 9-point stencil; life: the
-    plain kernel on small grids, bit tiles on wide ones."""
+    """Temporal tiles of 16 rows for the Jacobi and 9-point stencils at every
+    size (depth 4, or the sweep count when smaller); life: the plain kernel on
+    small grids, bit tiles on wide ones."""
     c = T.heuristic_choice(_q((125, 205), "f32", "jacobi2d", 158))
-    assert (c.tile_y, c.time_tile) == (8, 4)
+    assert (c.tile_y, c.time_tile) == (16, 4)
     c = T.heuristic_choice(_q((1310, 2751), "f32", "diffusion9", 324))
     assert (c.tile_y, c.time_tile) == (16, 4)
     assert T.heuristic_choice(_q((2903, 1983), "f32", "diffusion9", 3)).time_tile == 3
