@@ -82,3 +82,18 @@ def test_auto_gpu_uses_tuner(gpu):
     st = api.Stencil2D(inp, out, api.Mask2D(4), K.jacobi2d())
     st.runIterativeAutoGPU(17)
     assert np.array_equal(out.hostArray, oracle.run_2d(K.jacobi2d(), a, 17))
+
+
+def test_model_of_another_width_is_refused(gpu):
+    """A model whose input width is not the tuner's feature count is refused
+    with an error, not read at the wrong stride; the heuristic alone still
+    answers."""
+    q = T.Query((64, 64, 64), "f32", "jacobi3d", 10)
+    m = T.CostModel(sizes=(L.TUNER_NFEAT - 2, 8, 1))
+    try:
+        with pytest.raises(L.PSkelError):
+            m.choose(q)
+    finally:
+        m.close()
+    h = T.heuristic_choice(q)
+    assert T.cfg_key(h) in {T.cfg_key(c) for c in T.candidates(q)}

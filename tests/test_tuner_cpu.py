@@ -111,3 +111,13 @@ def test_random_queries_reproducible():
     assert a == b
     for q in a:
         assert all(s >= 32 for s in q.shape) and 10 <= q.iterations <= 1000
+
+
+def test_heuristic_picks_a_candidate():
+    """On random queries of every kind (the training distribution) the
+    heuristic's launch is one of the tuner's own candidates, so the trainer
+    can price it from the exhaustive table."""
+    for q in T.random_queries(200, 7) + [_q((512, 512, 512), s, "jacobi3d", it) for s in ("f32", "bf16", "f16")
+                                           for it in (1, 2, 1000)]:
+        h = T.heuristic_choice(q)
+        assert T.cfg_key(h) in {T.cfg_key(c) for c in T.candidates(q)}, (q, T.cfg_key(h))
