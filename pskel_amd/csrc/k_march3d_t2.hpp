@@ -136,6 +136,11 @@ __device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off,
     }
 }
 
+// Cache policy of the output stores: non-temporal (gfx950 `nt`, bit 1 of the
+// buffer intrinsics' aux operand).  No block of the launch reads the output
+// grid; measured 1.5% faster for f32 and 0.7% for bf16 at 512^3.
+constexpr int kStoreNT = 2;
+
 template <typename T, int VD>
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, const T (&v)[VD])
 {
@@ -144,15 +149,15 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
     if constexpr (B == 4) {
         uint32_t x;
         __builtin_memcpy(&x, v, 4);
-        __builtin_amdgcn_raw_buffer_store_b32(x, r, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(x, r, (int)off, 0, kStoreNT);
     } else if constexpr (B == 8) {
         u32x2_t x;
         __builtin_memcpy(&x, v, 8);
-        __builtin_amdgcn_raw_buffer_store_b64(x, r, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(x, r, (int)off, 0, kStoreNT);
     } else {
         u32x4_t x;
         __builtin_memcpy(&x, v, 16);
-        __builtin_amdgcn_raw_buffer_store_b128(x, r, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(x, r, (int)off, 0, kStoreNT);
     }
 }
 
