@@ -158,19 +158,20 @@ __device__ __forceinline__ void load_vec(const T* __restrict__ p, T (&out)[N])
 template <typename T, int N>
 __device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&in)[N])
 {
+    // non-temporal: a sweep's output grid is not read again within its launch
     constexpr int B = N * (int)sizeof(T);
     if constexpr (B == 16) {
-        uint4 x;
+        u32x4_t x;
         __builtin_memcpy(&x, in, 16);
-        *reinterpret_cast<uint4*>(p) = x;
+        __builtin_nontemporal_store(x, reinterpret_cast<u32x4_t*>(p));
     } else if constexpr (B == 8) {
-        uint2 x;
+        u32x2_t x;
         __builtin_memcpy(&x, in, 8);
-        *reinterpret_cast<uint2*>(p) = x;
+        __builtin_nontemporal_store(x, reinterpret_cast<u32x2_t*>(p));
     } else if constexpr (B == 4) {
         uint32_t x;
         __builtin_memcpy(&x, in, 4);
-        *reinterpret_cast<uint32_t*>(p) = x;
+        __builtin_nontemporal_store(x, reinterpret_cast<uint32_t*>(p));
     } else {
 #pragma unroll
         for (int i = 0; i < N; ++i) p[i] = in[i];
