@@ -294,7 +294,7 @@ pskel_status_t pskel_batch_destroy(pskel_batch_t* b);
  *      to; a learned cost model (pskel_mlp_t, below) predicts log time per
  *      sweep from pskel_tuner_features and the tuner takes the argmin.
  *      Exhaustive search over measured times (pskel_tuner_measure) is the
- *      fallback and the judge of the model's regret. */
+ *      fallback and the yardstick of the model's regret. */
 typedef struct pskel_tune_query {
     int32_t ndim;
     int32_t storage;     /* pskel_dtype_t of the grid */

@@ -14,8 +14,8 @@ objective and never timed anything.  Here:
   (grid dims, mask radius, storage type, iteration count, kernel family,
   candidate);
 * choose: argmin of the model's prediction over the candidates;
-* exhaustive / GA search over measured times: the fallback and the judge of the
-  model's regret.
+* exhaustive / GA search over measured times: the fallback and the yardstick
+  of the model's regret.
 """
 from __future__ import annotations
 
