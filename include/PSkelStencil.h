@@ -218,25 +218,37 @@ public:
         runIterativeGPU(1, GPUBlockSizeX, GPUBlockSizeY);
     }
 
-    // Plain contract: one lane per cell in (bx, by) blocks (:1047-1097, :1765-1792).
+    // N sweeps on the resident grid (:1047-1097).  "If GPUBlockSize is 0, the
+    // block size is automatically chosen" (include/PSkelStencil.h:252-256): with
+    // both block sizes 0 the launch comes from setLaunchConfig, else from the
+    // tuner (tuned kernels, temporal tiles), as runIterativeAutoGPU picks it for
+    // a resident grid.  Explicit block sizes keep the plain contract: one lane
+    // per cell in (bx, by) blocks (:1765-1792).
     void runIterativeGPU(size_t iterations, size_t GPUBlockSizeX = 0, size_t GPUBlockSizeY = 0)
     {
         pskel_launch_cfg_t cfg{};
+        if (GPUBlockSizeX == 0 && GPUBlockSizeY == 0) {
+            cfg.variant = PSKEL_VARIANT_TUNED;
+            gpuRun(iterations, cfg, false, /*autotune=*/true);
+            return;
+        }
         cfg.variant = PSKEL_VARIANT_PLAIN;
         cfg.block_x = (int)(GPUBlockSizeX ? GPUBlockSizeX : 64);
-        cfg.block_y = (int)(GPUBlockSizeY ? GPUBlockSizeY : (GPUBlockSizeX ? 1 : 4));
+        cfg.block_y = (int)(GPUBlockSizeY ? GPUBlockSizeY : 1);
         gpuRun(iterations, cfg, true);
     }
 
     // The reference's shared-memory pyramid (:943-1041): pyramidHeight sweeps
-    // fused per launch -- here overlapped LDS temporal tiles of that depth.
+    // fused per launch -- here overlapped temporal tiles of that depth: up to 4
+    // in 2D (8 for life), 2 in 3D (the fused 7-point pair; other 3D stencils
+    // sweep singly).
     void runIterativeGPU(size_t iterations, size_t pyramidHeight, size_t GPUBlockSizeX, size_t GPUBlockSizeY)
     {
         (void)GPUBlockSizeX;
         (void)GPUBlockSizeY;
         pskel_launch_cfg_t cfg{};
         cfg.variant = PSKEL_VARIANT_TUNED;
-        cfg.time_tile = dims() == 2 ? (int)std::max<size_t>(1, std::min<size_t>(pyramidHeight, 4)) : 1;
+        cfg.time_tile = (int)std::max<size_t>(1, std::min<size_t>(pyramidHeight, maxTemporalDepth()));
         gpuRun(iterations, cfg, false);
     }
 
@@ -259,12 +271,7 @@ public:
         }
         pskel_launch_cfg_t cfg{};
         cfg.variant = PSKEL_VARIANT_TUNED;
-        // temporal depth: up to 2 sweeps per launch in 3D (the 7-point Jacobi;
-        // other 3D stencils sweep singly), 4 in 2D, 8 for life
-        size_t kmax = dims() == 2 ? 4 : 2;
-        if constexpr (StencilTraits<Args>::has_descriptor)
-            if (dims() == 2 && StencilTraits<Args>::describe(mask, args, 2).kind == PSKEL_KIND_LIFE) kmax = 8;
-        cfg.time_tile = (int)std::max<size_t>(1, std::min<size_t>(innerIterations, kmax));
+        cfg.time_tile = (int)std::max<size_t>(1, std::min<size_t>(innerIterations, maxTemporalDepth()));
         gpuRun(iterations, cfg, false);
     }
 
@@ -389,6 +396,16 @@ public:
 protected:
     using T = std::remove_reference_t<decltype(*std::declval<Array&>().hostArray)>;
 
+    // Sweeps the library fuses per launch at most: 2 in 3D (the 7-point
+    // Jacobi; other 3D stencils sweep singly), 4 in 2D, 8 for life.
+    size_t maxTemporalDepth()
+    {
+        size_t kmax = dims() == 2 ? 4 : 2;
+        if constexpr (StencilTraits<Args>::has_descriptor)
+            if (dims() == 2 && StencilTraits<Args>::describe(mask, args, 2).kind == PSKEL_KIND_LIFE) kmax = 8;
+        return kmax;
+    }
+
     void ensureGrid()
     {
         detail::GpuState& s = *gpu;
@@ -439,6 +456,10 @@ protected:
             gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);  // user kernels stay resident
         } else {
             detail::GpuState& s = *gpu;
+            s.residuals.clear();
+            // host-staged passes reduce no residuals: refuse rather than
+            // return none for an interval that was asked for
+            if (s.residualEvery) gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);
             const int hdt = detail::hostDtype<T>();
             if (hdt == 0 || (s.storage != 0 && s.storage != hdt)) gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);
             pskel_grid_desc_t gd{};
@@ -564,6 +585,12 @@ protected:
             }
             if (hipGetLastError() != hipSuccess) gpuErrchk((pskel_status_t)PSKEL_ERR_LAUNCH);
             gpuErrchk(pskel_grid_swap(s.grid));
+            // FP64 ||u^k - u^(k-1)|| after every n-th sweep (on the grid's stream)
+            if (s.residualEvery && (it + 1) % s.residualEvery == 0) {
+                double r2 = 0.0;
+                gpuErrchk(pskel_grid_residual(s.grid, &r2));
+                s.residuals.push_back(r2);
+            }
         }
         (void)hipEventRecord(e1, stream);
         (void)hipEventSynchronize(e1);
@@ -573,11 +600,6 @@ protected:
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         m.deviceFree();
-        if (s.residualEvery) {
-            double r2 = 0.0;
-            gpuErrchk(pskel_grid_residual(s.grid, &r2));
-            s.residuals.assign(1, r2);
-        }
     }
 #endif
 #endif  // PSKEL_GPU

@@ -126,6 +126,8 @@ pskel_status_t pskel_batch_create(pskel_grid_t* const* grids, const pskel_stenci
 {
     if (!grids || !descs || !out || n == 0 || n > (1u << 20)) return fail(PSKEL_ERR_INVALID_ARGUMENT, "bad batch");
     *out = nullptr;
+    for (size_t i = 0; i < n; ++i)
+        if (!grids[i]) return fail(PSKEL_ERR_INVALID_ARGUMENT, "batch grid is NULL");
     auto* b = new pskel_batch_t();
     b->device = grids[0]->device;
     std::vector<int> items;

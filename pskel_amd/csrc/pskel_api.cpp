@@ -316,11 +316,18 @@ struct RunPlan {
 };
 
 // Sweeps fused in the next launch: up to the temporal depth, cut so that a launch
-// ends on every residual step.
-size_t next_k(size_t done, size_t iterations, int kmax, size_t every)
+// ends on every residual step.  The residual of sweep n is ||u^n - u^(n-1)||.
+// A 3D fused pair reduces it inside the launch (its second sweep against its
+// first); every other launch leaves only u^n and the state before the launch,
+// so in 2D a launch of k > 1 sweeps that would end on a residual step stops one
+// sweep short and the residual step runs as a single sweep.
+size_t next_k(size_t done, size_t iterations, int kmax, size_t every, int ndim)
 {
     size_t k = std::min<size_t>((size_t)kmax, iterations - done);
-    if (every) k = std::min(k, (done / every + 1) * every - done);
+    if (every) {
+        k = std::min(k, (done / every + 1) * every - done);
+        if (ndim == 2 && k > 1 && (done + k) % every == 0) k -= 1;
+    }
     return k;
 }
 
@@ -354,7 +361,7 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
     size_t done = 0;
     int64_t nl = 0, nr = 0;
     while (done < iterations) {
-        const size_t k = next_k(done, iterations, kmax, every);
+        const size_t k = next_k(done, iterations, kmax, every, ndim);
         const bool want_res = every && ((done + k) % every == 0);
         const void* src = g->buf[0];
         void* dst;
@@ -511,16 +518,14 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
             // life runs up to 8 generations per launch on bit-packed cells
             const int kmax = plan.pattern == PAT_LIFE8 ? 8 : 4;
             if (cfg.time_tile > kmax) return fail(PSKEL_ERR_UNSUPPORTED, "temporal depth above 4 (8 for life)");
-            if (cfg.residual_every > 0 && cfg.residual_every < cfg.time_tile)
-                return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
+            // a residual interval shorter than the depth only shortens the
+            // launches: none crosses a residual step (next_k)
             plan.path = 2;
         }
         if (cfg.time_tile > 2 && ndim == 3) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 2");
         // two fused sweeps per launch run the 7-point Jacobi; for any other 3D
         // stencil the depth is a hint and sweeps run singly
         if (ndim == 3 && cfg.time_tile == 2 && tunable) {
-            if (cfg.residual_every > 0 && cfg.residual_every < cfg.time_tile)
-                return fail(PSKEL_ERR_UNSUPPORTED, "residual interval shorter than the temporal depth");
             // the fused kernel addresses a plane through a buffer of < 2^31 bytes
             if ((uint64_t)g->W * g->pitch * g->elem > 0x7fffffffull)
                 return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal tiles need planes under 2 GiB");
@@ -537,6 +542,28 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
         if (bx <= 0 || by <= 0) return fail(PSKEL_ERR_INVALID_ARGUMENT, "block dims must be positive");
     }
     return PSKEL_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// FP64 partials a run with residuals needs: one per block of the largest
+// launch its sweeps use (3D sweeps reduce inside the stencil launch), or of
+// the separate difference kernel (2D).
+size_t residual_partials(const pskel_grid_t* g, const RunPlan& plan, const pskel_launch_cfg_t& cfg)
+{
+    size_t nb = diff_blocks(g->nrows);
+    if (g->ndim == 3) {
+        const int bx = cfg.block_x > 0 ? cfg.block_x : 64;
+        const int by = cfg.block_y > 0 ? cfg.block_y : 4;
+        Sweep3D s3 = make_sweep3d(g, nullptr, nullptr, plan.radius, nullptr);
+        pskel_launch_cfg_t one = cfg;
+        if (plan.path == 2) one.tile_x = one.tile_y = one.tile_z = one.block_x = one.block_y = 0;
+        nb = std::max(nb, plan.path >= 1 ? tuned3d_blocks(s3, g->storage, one) : plain3d_blocks(s3, bx, by));
+        if (plan.path == 2) nb = std::max(nb, ttile3d_blocks(s3, g->storage, cfg));
+    }
+    return nb;
 }
 
 }  // namespace
@@ -809,8 +836,6 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
     if ((st = plan_run(g, d, cfg, plan)) != PSKEL_OK) return st;
     const int ndim = g->ndim;
     const int radius = plan.radius;
-    const int bx = cfg.block_x > 0 ? cfg.block_x : 64;
-    const int by = cfg.block_y > 0 ? cfg.block_y : 4;
     if (stats) stats->sweeps = (int64_t)iterations;
     if (iterations == 0) return PSKEL_OK;
 
@@ -823,15 +848,7 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
     const size_t every = cfg.residual_every > 0 ? (size_t)cfg.residual_every : 0;
     const size_t nres_max = every ? iterations / every : 0;
     if (every) {
-        Sweep3D s3 = make_sweep3d(g, nullptr, nullptr, radius, nullptr);
-        size_t nb = diff_blocks(g->nrows);
-        if (ndim == 3) {
-            pskel_launch_cfg_t one = cfg;
-            if (plan.path == 2) one.tile_x = one.tile_y = one.tile_z = one.block_x = one.block_y = 0;
-            nb = std::max(nb, plan.path >= 1 ? tuned3d_blocks(s3, g->storage, one) : plain3d_blocks(s3, bx, by));
-            if (plan.path == 2) nb = std::max(nb, ttile3d_blocks(s3, g->storage, cfg));
-        }
-        if ((st = ensure_partials(g, nb)) != PSKEL_OK) return st;
+        if ((st = ensure_partials(g, residual_partials(g, plan, cfg))) != PSKEL_OK) return st;
         if ((st = ensure_res(g, nres_max)) != PSKEL_OK) return st;
     }
     if (plan.path == 2 && ndim == 3) {
@@ -917,7 +934,7 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
         // advance the host-side state exactly as the captured sequence did
         const int kmax = (plan.path == 2) ? std::max(1, cfg.time_tile) : 1;
         for (size_t done = 0; done < iterations;) {
-            const size_t k = next_k(done, iterations, kmax, every);
+            const size_t k = next_k(done, iterations, kmax, every, g->ndim);
             if (k % 2 == 0 && g->ndim == 2) g->spare_parity = g->parity;  // ring copy in the graph
             advance_state(*g, k);
             g->sweeps_done += k;

@@ -26,6 +26,10 @@ namespace {
 
 pskel_mlp_t* g_default_model = nullptr;
 std::once_flag g_model_once;
+// Inference stages inputs in the model's own device buffers and stream, so
+// concurrent choices (runIterativeAutoGPU from several host threads) take
+// turns on any model.
+std::mutex g_predict_mutex;
 
 // The shipped cost model lives next to the library: <libdir>/data/costmodel.bin.
 void load_shipped_model()
@@ -282,6 +286,7 @@ pskel_status_t pskel_tuner_heuristic(const pskel_tune_query_t* q, const pskel_st
 pskel_status_t pskel_tuner_set_default_model(pskel_mlp_t* model)
 {
     std::call_once(g_model_once, [] {});  // an explicit choice wins over the shipped file
+    std::lock_guard<std::mutex> lock(g_predict_mutex);
     g_default_model = model;
     return PSKEL_OK;
 }
@@ -294,6 +299,7 @@ pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_que
     if (!out) return fail(PSKEL_ERR_INVALID_ARGUMENT, "out is NULL");
     if (!model) {
         std::call_once(g_model_once, load_shipped_model);
+        std::lock_guard<std::mutex> lock(g_predict_mutex);
         model = g_default_model;
     }
     if (!model) {
@@ -305,7 +311,10 @@ pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_que
     const auto v = candidates(*q, *d);
     std::vector<float> x(v.size() * PSKEL_TUNER_NFEAT), y(v.size());
     for (size_t i = 0; i < v.size(); ++i) features(*q, *d, v[i], &x[i * PSKEL_TUNER_NFEAT]);
-    st = mlp_predict_host(model, x.data(), v.size(), y.data());
+    {
+        std::lock_guard<std::mutex> lock(g_predict_mutex);
+        st = mlp_predict_host(model, x.data(), v.size(), y.data());
+    }
     if (st != PSKEL_OK) return st;
     const size_t best = (size_t)(std::min_element(y.begin(), y.end()) - y.begin());
     *out = v[best];

@@ -3,10 +3,20 @@
 //   drive <kernel> <method> <W> <H> <D> <iters> <input.bin> <output0.bin> <result.bin>
 //
 // kernel: jacobi2d | diffusion9 | life | jacobi3d | user2d | user3d
-// method: cpu | seq | gpu | pyramid | tiling | auto
+// method: cpu | seq | gpu (32 x 4 blocks) | default (runIterativeGPU(N): no
+//         block sizes) | pyramid | tiling | auto | ...
 // Arrays are raw little-endian host arrays in Array2D/Array3D order
 // (float, or uint8 for life).  The program is written the way a PSkel user
 // writes one: Arrays, a Mask, an Args, Stencil2D/3D, one run method.
+//
+// jacobi3d also takes `gen:<seed>` for both input files (the field generated
+// here: uniform [0,1) from splitmix64((seed << 40) ^ i), top 24 bits, faces at
+// 1.0) and `-` for the result file (not written).
+// Environment: PSKEL_DRIVE_RESIDUAL=n sets setResidualInterval(n) and prints
+// "residuals r1 r2 ..." on stderr; PSKEL_DRIVE_TIMES prints "gpu_ms <device
+// time of the sweeps> wall_ms <whole run method, host copies included>".
+#include <chrono>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -116,10 +126,15 @@ static bool writef(const char* p, const T* src, size_t n)
 template <class St>
 static int runMethod(St& st, const std::string& method, size_t iters)
 {
+#ifdef PSKEL_GPU
+    if (const char* r = std::getenv("PSKEL_DRIVE_RESIDUAL")) st.setResidualInterval(std::strtoull(r, nullptr, 10));
+#endif
+    const auto t0 = std::chrono::steady_clock::now();
     if (method == "cpu") st.runIterativeCPU(iters, 0);
     else if (method == "seq") st.runIterativeSequential(iters);
 #ifdef PSKEL_GPU
     else if (method == "gpu") st.runIterativeGPU(iters, 32, 4);
+    else if (method == "default") st.runIterativeGPU(iters);
     else if (method == "pyramid") st.runIterativeGPU(iters, 2, 32, 4);
     else if (method == "tiling") st.runIterativeTilingGPU(iters, 256, 256, 1, 3);
     else if (method == "auto") st.runIterativeAutoGPU(iters);
@@ -137,6 +152,17 @@ static int runMethod(St& st, const std::string& method, size_t iters)
         std::fprintf(stderr, "unknown method %s\n", method.c_str());
         return 2;
     }
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+#ifdef PSKEL_GPU
+    if (std::getenv("PSKEL_DRIVE_TIMES")) std::fprintf(stderr, "gpu_ms %.6f wall_ms %.6f\n", st.lastGpuMillis(), wall);
+    if (std::getenv("PSKEL_DRIVE_RESIDUAL")) {
+        std::fprintf(stderr, "residuals");
+        for (double r : st.residuals()) std::fprintf(stderr, " %.17g", r);
+        std::fprintf(stderr, "\n");
+    }
+#else
+    if (std::getenv("PSKEL_DRIVE_TIMES")) std::fprintf(stderr, "wall_ms %.6f\n", wall);
+#endif
     return 0;
 }
 
@@ -154,17 +180,37 @@ static int run2d(Mask mask, Args args, const std::string& method, size_t W, size
     return 0;
 }
 
+// `gen:<seed>`: the Jacobi-3D field, uniform [0,1) with every face at 1.0
+static bool genJacobi3D(const char* spec, float* dst, size_t W, size_t H, size_t D)
+{
+    if (std::strncmp(spec, "gen:", 4) != 0) return false;
+    const uint64_t seed = std::strtoull(spec + 4, nullptr, 10);
+    for (size_t h = 0; h < H; ++h)
+        for (size_t w = 0; w < W; ++w)
+            for (size_t d = 0; d < D; ++d) {
+                const size_t i = (h * W + w) * D + d;
+                uint64_t z = ((seed << 40) ^ (uint64_t)i) + 0x9E3779B97F4A7C15ull;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                const bool face = h == 0 || w == 0 || d == 0 || h + 1 == H || w + 1 == W || d + 1 == D;
+                dst[i] = face ? 1.0f : (float)(z >> 40) * (1.0f / 16777216.0f);
+            }
+    return true;
+}
+
 template <class Mask, class Args>
 static int run3d(Mask mask, Args args, const std::string& method, size_t W, size_t H, size_t D, size_t iters,
                  char** files)
 {
     Array3D<float> in(W, H, D), out(W, H, D);
     const size_t n = W * H * D;
-    if (!readf(files[0], in.hostArray, n) || !readf(files[1], out.hostArray, n)) return 3;
+    if (!genJacobi3D(files[0], in.hostArray, W, H, D) && !readf(files[0], in.hostArray, n)) return 3;
+    if (!genJacobi3D(files[1], out.hostArray, W, H, D) && !readf(files[1], out.hostArray, n)) return 3;
     Stencil3D<Array3D<float>, Mask, Args> st(in, out, mask, args);
     const int rc = runMethod(st, method, iters);
     if (rc) return rc;
-    if (!writef(files[2], out.hostArray, n)) return 4;
+    if (std::strcmp(files[2], "-") != 0 && !writef(files[2], out.hostArray, n)) return 4;
     in.hostFree();
     out.hostFree();
     return 0;

@@ -31,8 +31,8 @@ def _want(stencil, a, iters):
 
 @pytest.mark.parametrize("binary", ["drive_gpu_gxx", "drive_gpu_hip"])
 @pytest.mark.parametrize("kernel,stencil,field", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("method", ["gpu", "pyramid", "tiling", "auto", "hosttiling", "hostauto", "partition",
-                                    "partition_small"])
+@pytest.mark.parametrize("method", ["gpu", "default", "pyramid", "tiling", "auto", "hosttiling", "hostauto",
+                                    "partition", "partition_small"])
 def test_dropin_gpu_methods(gpu, binary, kernel, stencil, field, method):
     a = field()
     for iters in (1, 6, 11):
@@ -103,4 +103,52 @@ def test_dropin_map_on_device(gpu, kernel, shape):
             want = (np.float32(0.5) * want * want + np.float32(0.125)).astype(np.float32)
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (kernel, iters)
     rc, _, err = dropin.run("drive_gpu_gxx", kernel, "gpu", arr, 2)
+    assert rc == 10 + 5 and "unsupported" in err
+
+
+def _residuals(err):
+    line = [l for l in err.splitlines() if l.startswith("residuals")][-1]
+    return np.array([float(x) for x in line.split()[1:]])
+
+
+@pytest.mark.parametrize("binary", ["drive_gpu_gxx", "drive_gpu_hip"])
+@pytest.mark.parametrize("method", ["default", "auto", "tiling", "pyramid"])
+def test_dropin_residual_interval_1(gpu, binary, method, monkeypatch):
+    """setResidualInterval(1) with the tuned methods (temporal depths 2 in 3D,
+    3 or 4 in 2D): every sweep's FP64 residual, and the result unchanged."""
+    monkeypatch.setenv("PSKEL_DRIVE_RESIDUAL", "1")
+    a = inputs.jacobi3d_field(40, seed=25)
+    iters = 7
+    rc, got, err = dropin.run(binary, "jacobi3d", method, a, iters)
+    assert rc == 0, err
+    want, want_res = oracle.run_3d(K.jacobi3d(), a, iters, residual_every=1)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), method
+    np.testing.assert_allclose(_residuals(err), want_res, rtol=1e-12, atol=0)
+    b = inputs.jacobi2d_field(130, 300, seed=26)
+    rc, got, err = dropin.run(binary, "jacobi2d", method, b, iters)
+    assert rc == 0, err
+    assert np.array_equal(got.view(np.uint8), oracle.run_2d(K.jacobi2d(), b, iters).view(np.uint8)), method
+    want_res = [np.linalg.norm(oracle.run_2d(K.jacobi2d(), b, n).astype(np.float64) -
+                               oracle.run_2d(K.jacobi2d(), b, n - 1).astype(np.float64)) for n in range(1, iters + 1)]
+    np.testing.assert_allclose(_residuals(err), want_res, rtol=1e-12, atol=0)
+
+
+def test_dropin_user_kernel_residual_interval(gpu, monkeypatch):
+    """A user stencilKernel (no descriptor) on the device: one residual every
+    n sweeps, as for the library's kernels."""
+    monkeypatch.setenv("PSKEL_DRIVE_RESIDUAL", "3")
+    a = inputs.jacobi3d_field(36, seed=27)
+    rc, got, err = dropin.run("drive_gpu_hip", "user3d", "gpu", a, 10)
+    assert rc == 0, err
+    want, want_res = oracle.run_3d(K.jacobi3d(), a, 10, residual_every=3)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+    np.testing.assert_allclose(_residuals(err), want_res, rtol=1e-12, atol=0)
+
+
+def test_dropin_host_staged_refuses_residuals(gpu, monkeypatch):
+    """Host-staged passes reduce no residuals: an interval that was asked for
+    is refused instead of silently returning none."""
+    monkeypatch.setenv("PSKEL_DRIVE_RESIDUAL", "2")
+    a = inputs.jacobi3d_field(30, seed=28)
+    rc, _, err = dropin.run("drive_gpu_gxx", "jacobi3d", "hosttiling", a, 4)
     assert rc == 10 + 5 and "unsupported" in err
