@@ -65,7 +65,9 @@ WORKLOADS = {
                      bytes_per_cu=16),
 }
 
-# Default tuned launch shapes (overridden by --tile / --time-tile).
+# Launch shapes of the workloads the tuner does not choose for: one sweep
+# per launch (the storage comparison), CLOUDSIM (field arguments).  The other
+# tuned rows take pskel_tuner_choose's pick (--tile / --time-tile override).
 TUNED = {
     # two fused sweeps per launch: 18-row x 512-cell tiles of 2 x 6 waves,
     # segment length chosen from the device's resident blocks
@@ -106,17 +108,66 @@ def field_of(name):
 
 
 def store_offset_of(name):
-    # 16-bit storage holds u - 0.5, the mean of the initial field: over 1000
-    # sweeps of a 512^3 grid most cells stay near it (the faces' pull towards
-    # 1.0 reaches a few tens of cells), so the stored deviations are small and
-    # keep more absolute precision
-    return 0.5 if name.startswith(("jacobi3d_bf16", "jacobi3d_f16")) else 0.0
+    # 16-bit storage holds u - offset with the offset set automatically to the
+    # mean of the uploaded field (about 0.5 here): over 1000 sweeps of a 512^3
+    # grid most cells stay near it (the faces' pull towards 1.0 reaches a few
+    # tens of cells), so the stored deviations are small and keep more
+    # absolute precision
+    return "auto" if name.startswith(("jacobi3d_bf16", "jacobi3d_f16")) else 0.0
 
 
 def rounding_of(name):
     # 16-bit stores round stochastically: nearest rounding stalls the slow
     # relaxation once updates drop below half an ulp
     return "stochastic" if name.startswith(("jacobi3d_bf16", "jacobi3d_f16")) else "nearest"
+
+
+# ---------------------------------------------------------------------------
+# the tuner's choices and their regret (measured, this device)
+# ---------------------------------------------------------------------------
+TUNER_WORKLOADS = ("jacobi3d", "jacobi3d_bf16", "jacobi3d_f16", "life", "jacobi2d", "diffusion9")
+TUNER_QUERY = {"jacobi3d": "jacobi3d", "jacobi3d_bf16": "jacobi3d", "jacobi3d_f16": "jacobi3d", "life": "life",
+               "jacobi2d": "jacobi2d", "diffusion9": "diffusion9"}
+
+
+def tuner_pick(name):
+    """pskel_tuner_choose's launch for the workload (the shipped cost model)."""
+    import ctypes as C
+
+    from pskel_amd import _lib as L
+    from pskel_amd import tuner as T
+
+    w = WORKLOADS[name]
+    q = T.Query(w["shape"], w["storage"], TUNER_QUERY[name], w["sweeps"])
+    cq, d = q.c_query(), stencil_of(name).desc()
+    out = L.LaunchCfg()
+    L.check(L.lib().pskel_tuner_choose(None, C.byref(cq), C.byref(d), C.byref(out)))
+    return out
+
+
+def tuner_regret(name, pick, sweeps=24, reps=3):
+    """Exhaustive search over the workload's candidates on its own grid
+    (pskel_tuner_search, `sweeps` sweeps per timing, median of `reps`), and the
+    tuner's pick timed the same way: regret = pick / best - 1."""
+    import ctypes as C
+
+    from pskel_amd import _lib as L
+    from pskel_amd import api
+    from pskel_amd import tuner as T
+
+    w = WORKLOADS[name]
+    d = stencil_of(name).desc()
+    with api.Grid(w["shape"], w["storage"], store_offset=store_offset_of(name), rounding=rounding_of(name),
+                  seed=17) as g:
+        g.upload_pair(field_of(name))
+        best, ms, nm = L.LaunchCfg(), C.c_double(), C.c_int32()
+        L.check(L.lib().pskel_tuner_search(g._h, C.byref(d), w["sweeps"], L.SEARCH_EXHAUSTIVE, sweeps, reps,
+                                           C.byref(best), C.byref(ms), C.byref(nm)))
+        tp = C.c_double()
+        L.check(L.lib().pskel_tuner_measure(g._h, C.byref(d), sweeps, C.byref(pick), reps, C.byref(tp)))
+    return {"pick": T.cfg_dict(pick), "pick_ms_per_sweep": tp.value, "best": T.cfg_dict(best),
+            "best_ms_per_sweep": ms.value, "candidates_timed": nm.value,
+            "regret": tp.value / ms.value - 1.0}
 
 
 def measure(name, variant, steps, warmup, field=None, run_kw=None):
@@ -127,7 +178,12 @@ def measure(name, variant, steps, warmup, field=None, run_kw=None):
     w = WORKLOADS[name]
     kw = dict(run_kw or {})
     if variant == "tuned" and not kw:
-        kw = dict(TUNED[name])
+        if name in TUNER_WORKLOADS:
+            from pskel_amd import tuner as T
+
+            kw = T.cfg_dict(tuner_pick(name))
+        else:
+            kw = dict(TUNED[name])
     if w.get("residual_every"):
         kw["residual_every"] = w["residual_every"]
     f = field_of(name) if field is None else field
@@ -188,17 +244,66 @@ def hbm_frac(r):
     return r["algorithmic_bytes_per_cu"] * r["cells"] / (r["avg_launch_ms"] * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
 
-def measure_host_tiled(sweeps=64, tile=64, inner=8, pcie_duplex_gbs=48.6):
+def measure_dropin(sweeps=1000, every=50):
+    """Jacobi-3D 512^3 x1000 through the C++ drop-in headers: a PSkel program
+    (tests/cpp/drive.cpp built by g++) calling Stencil3D::runIterativeGPU(N)
+    with no block sizes (the tuner's launch), FP64 residual every 50 sweeps;
+    the second of two calls is reported.  gpu_ms: device time of the sweeps
+    (HIP events); wall_ms: the whole call, both host arrays uploaded and the
+    result downloaded over PCIe as the reference's runIterativeGPU does."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", "drive_gpu_gxx")
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, PSKEL_DRIVE_TIMES="1", PSKEL_DRIVE_RESIDUAL=str(every), PSKEL_DRIVE_REPEAT="2")
+    p = subprocess.run([exe, "jacobi3d", "default", "512", "512", "512", str(sweeps), "gen:4", "gen:4", "-"],
+                       capture_output=True, text=True, env=env, timeout=900)
+    if p.returncode != 0:
+        return {"error": p.stderr[-500:]}
+    times = re.findall(r"gpu_ms ([0-9.eE+-]+) wall_ms ([0-9.eE+-]+)", p.stderr)
+    res = [l for l in p.stderr.splitlines() if l.startswith("residuals")]
+    gpu_ms, wall_ms = (float(x) for x in times[-1])
+    cu = 512 ** 3 * sweeps
+    return {"workload": f"jacobi3d-7pt 512^3 f32 x{sweeps}, FP64 residual every {every}, "
+                        "Stencil3D::runIterativeGPU(N) through the C++ drop-in (g++ build)",
+            "gpu_ms": gpu_ms, "wall_ms": wall_ms, "cell_updates_per_s_device": cu / (gpu_ms * 1e-3),
+            "cell_updates_per_s_with_pcie": cu / (wall_ms * 1e-3),
+            "residuals": len(res[-1].split()) - 1 if res else 0}
+
+
+def host_search(f, s, sweeps):
+    """(slab height, sweeps per pass) from pskel_tuner_host_search."""
+    import ctypes as C
+
+    from pskel_amd import _lib as L
+
+    gd = L.GridDesc()
+    gd.ndim = 3
+    gd.storage = L.DTYPE_F32
+    gd.height, gd.width, gd.depth = f.shape
+    d = s.desc()
+    tile, inner, ms = C.c_size_t(), C.c_size_t(), C.c_double()
+    L.check(L.lib().pskel_tuner_host_search(C.byref(gd), f.ctypes.data, L.DTYPE_F32, C.byref(d), sweeps, 0,
+                                            C.byref(tile), C.byref(inner), C.byref(ms)))
+    return tile.value, inner.value
+
+
+def measure_host_tiled(sweeps=64, tile=None, inner=None, pcie_duplex_gbs=48.6):
     """Jacobi-3D 512^3 f32 out of core: the grid stays in host memory, slabs of
     `tile` h-planes plus an `inner`-plane halo visit the device for `inner`
-    sweeps at a time.  The time is the library call's own (its host copies in
-    and out included).  The bound is the PCIe traffic of the passes at the
-    duplex rate one MI355X link sustains (48.6 GB/s each way, pinned buffers,
-    H2D and D2H at once)."""
+    sweeps at a time, (tile, inner) from the library's out-of-core search
+    unless given.  The time is the library call's own (its host copies in and
+    out included).  The bound is the PCIe traffic of the passes at the duplex
+    rate one MI355X link sustains (48.6 GB/s each way, pinned buffers, H2D and
+    D2H at once)."""
     from pskel_amd import api
 
     f = field_of("jacobi3d")
     s = stencil_of("jacobi3d")
+    searched = tile is None
+    if searched:
+        tile, inner = host_search(f, s, sweeps)
     api.run_host_tiled(s, f, inner, tile, inner)  # warm-up
     _, st = api.run_host_tiled(s, f, sweeps, tile, inner)
     wall = st["elapsed_ms"] / 1e3
@@ -218,6 +323,7 @@ def measure_host_tiled(sweeps=64, tile=64, inner=8, pcie_duplex_gbs=48.6):
         "cell_updates_per_s": cu / wall,
         "ms_per_sweep": wall * 1e3 / sweeps,
         "launches": st["launches"],
+        "slab_planes_and_inner_from": "pskel_tuner_host_search" if searched else "given",
         "pcie_bytes": {"h2d": up, "d2h": down},
         "pcie_bound_ms": bound * 1e3,
         "pcie_bound_frac": bound / wall,
@@ -239,30 +345,88 @@ def batch_fields(seed=5):
     return shapes, fields
 
 
-def batch_rows(shapes, sweeps):
-    """Rows per lane per grid from the cost model (restricted to the batch
-    kernel's shapes: 4 waves, 1/2/4/8 rows), 4 when no model is installed."""
+# batch-supported launch shapes per grid kind: depth 1 register tiles (rows per
+# lane), f32 LDS temporal tiles of 3 sweeps (16 or 32 rows), bit-packed life
+# tiles of 3, 5 or 7 generations
+def batch_candidates(kind):
+    from pskel_amd import _lib as L
+
+    def cfg(ty, tz, k):
+        c = L.LaunchCfg()
+        c.variant = L.VARIANT_TUNED
+        c.tile_y, c.tile_z, c.time_tile = ty, tz, k
+        return c
+
+    out = [cfg(ry, 4, 1) for ry in (1, 2, 4, 8)]
+    if kind == "life":
+        out += [cfg(32, 0, k) for k in (3, 5, 7)]
+    else:
+        out += [cfg(cy, 0, 3) for cy in (16, 32)]
+    return out
+
+
+def batch_query(h, w, kind, sweeps):
+    from pskel_amd import tuner as T
+
+    return T.Query((h, w), "u8" if kind == "life" else "f32", "life" if kind == "life" else "jacobi2d", sweeps)
+
+
+def batch_choices(shapes, sweeps):
+    """Per grid: the batch-supported launch shape with the lowest predicted
+    time under the shipped cost model (None without a model)."""
     from pskel_amd import tuner as T
 
     model = T.default_model()
     if model is None:
-        return [4] * len(shapes), "default (4 rows per lane)"
-    rows = []
+        return None
+    out = []
     for h, w, kind in shapes:
-        q = T.Query((h, w), "u8" if kind == "life" else "f32", "life" if kind == "life" else "jacobi2d", sweeps)
-        cands = [c for c in T.candidates(q) if c.variant == 1 and c.time_tile <= 1 and c.tile_z == 4
-                 and c.tile_y in (1, 2, 4, 8)]
-        pred = model.predict(T.features(q, cands))
-        rows.append(int(cands[int(np.argmin(pred))].tile_y))
-    return rows, "cost model"
+        cands = batch_candidates(kind)
+        pred = model.predict(T.features(batch_query(h, w, kind, sweeps), cands))
+        out.append(cands[int(np.argmin(pred))])
+    return out
 
 
-def measure_batch(steps, warmup, sweeps=200):
-    """256 uneven grids, `sweeps` sweeps per step: one persistent kernel per
-    sweep, captured in one hipGraph; beside it the same work launched grid by
-    grid (one runIterativeGPU per grid, the reference's way)."""
+def batch_regret_sample(shapes, picks, sweeps, nsample=16, reps=3):
+    """Regret of the model's batch choices on a sample of the grids: each
+    sampled grid alone, every batch-supported shape timed (pskel_tuner_measure)."""
+    import ctypes as C
+
+    from pskel_amd import _lib as L
     from pskel_amd import api
     from pskel_amd import kernels as K
+    from pskel_amd import tuner as T
+
+    idx = np.linspace(0, len(shapes) - 1, nsample).astype(int)
+    rows = []
+    for i in idx:
+        h, w, kind = shapes[i]
+        q = batch_query(h, w, kind, sweeps)
+        s = K.life() if kind == "life" else K.jacobi2d()
+        d = s.desc()
+        with api.Grid((h, w), q.storage) as g:
+            g.upload_pair(T.field_for(q))
+            t = []
+            for c in batch_candidates(kind):
+                ms = C.c_double()
+                L.check(L.lib().pskel_tuner_measure(g._h, C.byref(d), 24, C.byref(c), reps, C.byref(ms)))
+                t.append(ms.value)
+        keys = [T.cfg_key(c) for c in batch_candidates(kind)]
+        tp = t[keys.index(T.cfg_key(picks[i]))]
+        rows.append({"grid": [h, w, kind], "pick": T.cfg_dict(picks[i]), "regret": tp / min(t) - 1.0,
+                     "depth1_best_over_best": min(t[:4]) / min(t)})
+    return rows
+
+
+def measure_batch(steps, warmup, sweeps=200, compare=True):
+    """256 uneven grids, `sweeps` sweeps per step: persistent kernels per
+    launch, captured in one hipGraph, with the launch shape and temporal depth
+    of every grid from the cost model; beside it the same batch at depth 1
+    (register tiles, the model's rows per lane) and the same work launched
+    grid by grid (one runIterativeGPU per grid, the reference's way)."""
+    from pskel_amd import api
+    from pskel_amd import kernels as K
+    from pskel_amd import tuner as T
 
     shapes, fields = batch_fields()
     grids, stencils = [], []
@@ -271,52 +435,85 @@ def measure_batch(steps, warmup, sweeps=200):
         g.upload_pair(f)
         grids.append(g)
         stencils.append(K.life() if kind == "life" else K.jacobi2d())
-    rows, how = batch_rows(shapes, sweeps)
+    picks = batch_choices(shapes, sweeps)
+    how = "cost model" if picks is not None else "default (depth 1, 4 rows per lane)"
+    if picks is None:
+        rows, depths = [4] * len(shapes), [1] * len(shapes)
+        rows1 = rows
+    else:
+        rows = [int(c.tile_y) for c in picks]
+        depths = [max(1, int(c.time_tile)) for c in picks]
+        # depth 1 beside it: the model's best register tile per grid
+        rows1 = []
+        model = T.default_model()
+        for h, w, kind in shapes:
+            cands = batch_candidates(kind)[:4]
+            pred = model.predict(T.features(batch_query(h, w, kind, sweeps), cands))
+            rows1.append(int(cands[int(np.argmin(pred))].tile_y))
     cells = sum(h * w for h, w, _ in shapes)
     nbytes = sum(h * w * (1 if k == "life" else 4) * 2 for h, w, k in shapes)
-    try:
-        with api.Batch(grids, stencils, rows) as b:
+
+    def timed(rows_, depths_):
+        with api.Batch(grids, stencils, rows_, depths_) as b:
             for _ in range(warmup):
                 b.run(sweeps, use_graph=True)
             api.synchronize()
             t0 = time.perf_counter()
             dev = 0.0
+            launches = 0
             for _ in range(steps):
-                dev += b.run(sweeps, use_graph=True)["elapsed_ms"]
+                st = b.run(sweeps, use_graph=True)
+                dev += st["elapsed_ms"]
+                launches = st["launches"]
             api.synchronize()
-            wall = time.perf_counter() - t0
-            info = b.info()
-        # separate launches: every grid on its own, tuned kernel, no graph
-        t1 = time.perf_counter()
-        for g, s in zip(grids, stencils):
-            g.run(s, sweeps, variant="tuned")
-        api.synchronize()
-        sep = time.perf_counter() - t1
+            return time.perf_counter() - t0, dev, launches, b.info()
+
+    try:
+        wall, dev, launches, info = timed(rows, depths)
+        if compare:
+            wall1, _, launches1, _ = timed(rows1, [1] * len(shapes))
+            # separate launches: every grid on its own, tuned kernel, no graph
+            t1 = time.perf_counter()
+            for g, s in zip(grids, stencils):
+                g.run(s, sweeps, variant="tuned")
+            api.synchronize()
+            sep = time.perf_counter() - t1
     finally:
         for g in grids:
             g.close()
-    return {
+    out = {
         "workload": f"batch 256 grids (sides 64..2048) jacobi f32 + life u8 x{sweeps}, one hipGraph",
         "cell_updates_per_s": cells * sweeps * steps / wall,
         "ms_per_step": wall * 1e3 / steps,
         "device_ms_per_step": dev / steps,
-        "hbm_roofline_frac": nbytes * sweeps * steps / wall / (HBM_PEAK_GBS * 1e9),
-        "separate_launches_ms": sep * 1e3,
-        "speedup_vs_separate": sep / (wall / steps),
+        "launch_steps": launches,
+        "hbm_roofline_frac_per_sweep_bytes": nbytes * sweeps * steps / wall / (HBM_PEAK_GBS * 1e9),
+        "depths": {str(k): depths.count(k) for k in sorted(set(depths))},
         "items": info["items"],
         "persistent_blocks": info["blocks"],
-        "rows_per_lane_from": how,
+        "launch_shapes_from": how,
         "cells": cells,
     }
+    if not compare:
+        return out
+    out.update({"depth1_ms_per_step": wall1 * 1e3 / steps, "depth1_launch_steps": launches1,
+                "speedup_vs_depth1": wall1 / wall, "separate_launches_ms": sep * 1e3,
+                "speedup_vs_separate": sep / (wall / steps)})
+    if picks is not None:
+        reg = batch_regret_sample(shapes, picks, sweeps)
+        out["regret_sample"] = {"grids": len(reg), "mean": float(np.mean([r["regret"] for r in reg])),
+                                "max": float(np.max([r["regret"] for r in reg])), "per_grid": reg}
+    return out
 
 
 def accuracy_3d(sweeps=1000, every=50, n=512):
     """Jacobi-3D n^3 x sweeps against the FP64 CPU oracle: f32 storage, and bf16
-    and f16 storage of u - 0.5 with stochastic rounding (f32 arithmetic, tuned
-    kernels), plus three bf16 controls (nearest rounding; offset 0.75; no
-    offset).  Reports the max absolute
-    and relative-L2 error of each GPU result and the FP64 residuals the device
-    reduces every `every` sweeps next to the oracle's."""
+    and f16 storage of u - offset with stochastic rounding and the automatic
+    offset (the mean of the uploaded field; f32 arithmetic, tuned kernels),
+    plus three bf16 controls (nearest rounding; offset 0.75; no offset).
+    Reports the max absolute and relative-L2 error of each GPU result and the
+    FP64 residuals the device reduces every `every` sweeps next to the
+    oracle's."""
     import oracle
     from pskel_amd import api
     from pskel_amd import inputs
@@ -331,13 +528,14 @@ def accuracy_3d(sweeps=1000, every=50, n=512):
     norm = float(np.linalg.norm(want))
     out = {"oracle": f"FP64 runIterativeCPU restatement, {threads} threads, {oracle_s:.0f} s",
            "grid": f"{n}^3 x {sweeps}", "residual_every": every}
-    for name, storage, off, rnd in (("f32", "f32", 0.0, "nearest"), ("bf16", "bf16", 0.5, "stochastic"),
-                                    ("f16", "f16", 0.5, "stochastic"), ("bf16_nearest", "bf16", 0.5, "nearest"),
+    for name, storage, off, rnd in (("f32", "f32", 0.0, "nearest"), ("bf16", "bf16", "auto", "stochastic"),
+                                    ("f16", "f16", "auto", "stochastic"), ("bf16_nearest", "bf16", "auto", "nearest"),
                                     ("bf16_offset_0.75", "bf16", 0.75, "stochastic"),
                                     ("bf16_no_offset", "bf16", 0.0, "stochastic")):
         kw = dict(TUNED["jacobi3d" if storage == "f32" else "jacobi3d_" + storage])
         with api.Grid(f32.shape, storage, store_offset=off, rounding=rnd, seed=17) as g:
             g.upload_pair(f32)
+            off = g.info()["store_offset"] if storage != "f32" else 0.0
             st = g.run(K.jacobi3d(), sweeps, variant="tuned", residual_every=every, **kw)
             got = g.download(np.float64)
         d = got - want
@@ -464,6 +662,8 @@ def main(argv=None):
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64-oracle error report")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the plain copy kernel on two buffers of this size (PMC calibration)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 256-grid batch with the model's launch shapes (no comparisons)")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -480,6 +680,10 @@ def main(argv=None):
 
     if args.copy_mib:
         return copy_calibration(args.copy_mib, max(args.steps, 1))
+    if args.batch_only:
+        # the batch alone, with the model's depths (a profiling target)
+        print(json.dumps(measure_batch(max(args.steps, 1), args.warmup, compare=False)))
+        return 0
 
     run_kw = None
     if args.tile or args.time_tile:
@@ -518,7 +722,21 @@ def main(argv=None):
         rb = measure_batch(max(1, min(args.steps, 3)), 1)
         extra["batch256:graph"] = rb
         print(f"# batch256: {rb['cell_updates_per_s']:.4g} CU/s, {rb['ms_per_step']:.2f} ms/step, "
-              f"{rb['speedup_vs_separate']:.1f}x vs separate launches", file=sys.stderr, flush=True)
+              f"{rb['speedup_vs_depth1']:.2f}x vs depth 1, {rb['speedup_vs_separate']:.1f}x vs separate launches, "
+              f"depths {rb['depths']}, sample regret "
+              f"{rb.get('regret_sample', {}).get('mean', float('nan')):.3f}", file=sys.stderr, flush=True)
+        rd = measure_dropin()
+        extra["jacobi3d:dropin"] = rd
+        if rd and "gpu_ms" in rd:
+            print(f"# jacobi3d drop-in runIterativeGPU(1000): {rd['cell_updates_per_s_device']:.4g} CU/s device, "
+                  f"{rd['cell_updates_per_s_with_pcie']:.4g} CU/s with PCIe", file=sys.stderr, flush=True)
+        # the tuner: its pick for each workload against exhaustive search
+        tuner = {}
+        for nm in TUNER_WORKLOADS:
+            tuner[nm] = tuner_regret(nm, tuner_pick(nm))
+            print(f"# tuner {nm}: pick {tuner[nm]['pick']} regret {tuner[nm]['regret']:.3f} "
+                  f"(best {tuner[nm]['best']})", file=sys.stderr, flush=True)
+        extra["tuner_regret_vs_exhaustive"] = tuner
 
     # HBM bytes one launch must move: one read and one write of the grid (plus
     # field reads), whatever the number of sweeps it fuses
@@ -533,7 +751,7 @@ def main(argv=None):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profiles(args.workload),
-        "kernel": (("k3d_march2" if (run_kw or TUNED[args.workload]).get("time_tile", 1) == 2 else "k3d_march")
+        "kernel": (("k3d_march2" if head["launch"].get("time_tile", 1) == 2 else "k3d_march")
                    if args.workload.startswith("jacobi3d") and args.variant == "tuned" else None),
         "sweeps_per_launch": round(head["sweeps_per_launch"], 3),
         "algorithmic_bytes_per_launch": bytes_per_launch,

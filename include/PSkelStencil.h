@@ -288,15 +288,12 @@ public:
     {
         (void)GPUBlockSize;
         if (gpu->hostTiling || !fitsDevice()) {
-            // out of core (:2014-2074): the largest slab whose device copies
-            // (3 slots of up to 3 buffers) fit in half the free memory, 8
-            // sweeps per pass
-            const size_t inner = 8, halo = mask.getRange() * inner;
-            size_t freeb = 0, totalb = 0;
-            gpuErrchk(pskel_device_info(gpu->device, nullptr, 0, &freeb, &totalb, nullptr));
-            const size_t plane = input.getWidth() * (dims() == 3 ? input.getDepth() : 1) * sizeof(T);
-            const size_t slab = freeb / 2 / (9 * std::max<size_t>(plane, 1));
-            hostTiledRun(iterations, slab > 2 * halo + 1 ? slab - 2 * halo : 1, inner);
+            // out of core (:2014-2074): slab height and sweeps per pass from a
+            // measured search over the pipeline's candidates that fit in half
+            // the free device memory (pskel_tuner_host_search)
+            size_t tile = 0, inner = 0;
+            hostTiledSearch(iterations, tile, inner);
+            hostTiledRun(iterations, tile, inner);
             return;
         }
         pskel_launch_cfg_t cfg{};
@@ -372,9 +369,11 @@ public:
     // slabs) even when the grid fits in device memory.
     void setHostTiling(bool on) { gpu->hostTiling = on; }
     // Device storage of the grid: PSKEL_DTYPE_BF16 / F16 keep (u - offset) in
-    // 16 bits with f32 arithmetic (3D); 0 = the host element type.  `rounding`
-    // is PSKEL_ROUND_NEAREST or PSKEL_ROUND_STOCHASTIC (reproducible from `seed`).
-    void setDeviceStorage(int storage, float offset = 0.0f, int rounding = PSKEL_ROUND_NEAREST,
+    // 16 bits with f32 arithmetic (3D); 0 = the host element type.  The offset
+    // defaults to PSKEL_STORE_OFFSET_AUTO: the mean of `input` at each run.
+    // `rounding` is PSKEL_ROUND_NEAREST or PSKEL_ROUND_STOCHASTIC
+    // (reproducible from `seed`).
+    void setDeviceStorage(int storage, float offset = PSKEL_STORE_OFFSET_AUTO, int rounding = PSKEL_ROUND_NEAREST,
                           uint32_t seed = 0)
     {
         gpu->storage = storage;
@@ -413,8 +412,9 @@ protected:
         if (storage == PSKEL_DTYPE_F64 || storage == 0)
             gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);  // no FP64 sweeps on the device
         const size_t d0 = input.getWidth(), d1 = input.getHeight(), d2 = input.getDepth();
+        const bool sameOffset = s.gridOffset == s.storeOffset || (std::isnan(s.gridOffset) && std::isnan(s.storeOffset));
         if (s.grid && (s.dims[0] != d0 || s.dims[1] != d1 || s.dims[2] != d2 || s.gridStorage != storage ||
-                       s.gridRounding != s.rounding || s.gridSeed != s.seed || s.gridOffset != s.storeOffset)) {
+                       s.gridRounding != s.rounding || s.gridSeed != s.seed || !sameOffset)) {
             pskel_grid_destroy(s.grid);
             s.grid = nullptr;
         }
@@ -447,6 +447,29 @@ protected:
         gpuErrchk(pskel_device_info(gpu->device, nullptr, 0, &freeb, &totalb, nullptr));
         const size_t bytes = input.size() * sizeof(T);
         return 3 * bytes + (size_t(64) << 20) < freeb;
+    }
+
+    // The out-of-core launch (slab height, sweeps per pass) for this grid.
+    void hostTiledSearch(size_t iterations, size_t& tile, size_t& inner)
+    {
+        if constexpr (Array::kDims == 1 || !StencilTraits<Args>::has_descriptor) {
+            gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);  // user kernels stay resident
+        } else {
+            detail::GpuState& s = *gpu;
+            const int hdt = detail::hostDtype<T>();
+            if (hdt == 0 || (s.storage != 0 && s.storage != hdt)) gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);
+            pskel_grid_desc_t gd{};
+            gd.ndim = dims();
+            gd.storage = hdt;
+            gd.width = input.getWidth();
+            gd.height = input.getHeight();
+            gd.depth = dims() == 3 ? input.getDepth() : 1;
+            gd.device = s.device;
+            pskel_stencil_desc_t desc = StencilTraits<Args>::describe(mask, args, dims());
+            std::vector<T> in(input.size());
+            input.packTo(in.data());
+            gpuErrchk(pskel_tuner_host_search(&gd, in.data(), hdt, &desc, iterations, 0, &tile, &inner, nullptr));
+        }
     }
 
     // Out-of-core sweeps through the library (descriptor kernels only).
@@ -493,11 +516,14 @@ protected:
         const int hdt = detail::hostDtype<T>();
         if (hdt == 0) gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);
         // upload both buffers: the device state starts as the host state
-        std::vector<T> buf(input.size());
+        std::vector<T> buf(input.size()), obuf(output.size());
         input.packTo(buf.data());
-        gpuErrchk(pskel_grid_upload(s.grid, PSKEL_BUFFER_INPUT, buf.data(), hdt));
-        output.packTo(buf.data());
-        gpuErrchk(pskel_grid_upload(s.grid, PSKEL_BUFFER_OUTPUT, buf.data(), hdt));
+        output.packTo(obuf.data());
+        auto upload = [&]() {
+            gpuErrchk(pskel_grid_upload(s.grid, PSKEL_BUFFER_INPUT, buf.data(), hdt));
+            gpuErrchk(pskel_grid_upload(s.grid, PSKEL_BUFFER_OUTPUT, obuf.data(), hdt));
+        };
+        upload();
         if (!plain && s.haveCfg) cfg = s.cfg;
         cfg.residual_every = (int)s.residualEvery;
         s.residuals.clear();
@@ -512,16 +538,26 @@ protected:
                 for (int k = 0; k < StencilTraits<Args>::num_fields; ++k)
                     gpuErrchk(pskel_grid_set_field(s.grid, k, fl[k]));
             }
-            if (autotune && !s.haveCfg) {
-                pskel_tune_query_t q{};
-                q.ndim = dims();
-                q.width = input.getWidth();
-                q.height = input.getHeight();
-                q.depth = dims() == 3 ? input.getDepth() : 1;
-                q.storage = s.gridStorage;
-                q.iterations = iterations;
+            if (autotune && !s.haveCfg && iterations > 0) {
                 pskel_launch_cfg_t tuned{};
-                gpuErrchk(pskel_tuner_choose(nullptr, &q, &desc, &tuned));
+                int32_t haveModel = 0;
+                gpuErrchk(pskel_tuner_has_model(&haveModel));
+                if (haveModel) {
+                    pskel_tune_query_t q{};
+                    q.ndim = dims();
+                    q.width = input.getWidth();
+                    q.height = input.getHeight();
+                    q.depth = dims() == 3 ? input.getDepth() : 1;
+                    q.storage = s.gridStorage;
+                    q.iterations = iterations;
+                    gpuErrchk(pskel_tuner_choose(nullptr, &q, &desc, &tuned));
+                } else {
+                    // no cost model: time the candidates on this grid (once per
+                    // shape in the process), then start again from the host state
+                    gpuErrchk(pskel_tuner_search(s.grid, &desc, iterations, PSKEL_SEARCH_EXHAUSTIVE, 0, 3, &tuned,
+                                                 nullptr, nullptr));
+                    upload();
+                }
                 tuned.residual_every = cfg.residual_every;
                 cfg = tuned;
             }

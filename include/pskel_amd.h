@@ -137,13 +137,17 @@ typedef enum pskel_rounding {
     PSKEL_ROUND_STOCHASTIC = 1
 } pskel_rounding_t;
 
+/* store_offset of a grid whose 16-bit storage offset follows its data */
+#define PSKEL_STORE_OFFSET_AUTO (__builtin_nanf(""))
+
 typedef struct pskel_grid_desc {
     int32_t ndim;        /* 2 or 3 */
     int32_t storage;     /* pskel_dtype_t on the device: U8, F32, BF16 or F16 */
     size_t width;        /* Array2D/Array3D width  (include/PSkelArray.h:295-384) */
     size_t height;       /* height */
     size_t depth;        /* depth (1 for 2D); the contiguous axis in 3D (include/PSkelArray.hpp:642) */
-    float store_offset;  /* BF16/F16 storage holds (u - store_offset) */
+    float store_offset;  /* BF16/F16 storage holds (u - store_offset); NaN (PSKEL_STORE_OFFSET_AUTO):
+                            each INPUT upload of F32/F64 data sets it to the data's mean */
     int32_t device;      /* HIP device ordinal */
     int32_t rounding;    /* pskel_rounding_t of BF16/F16 stores */
     uint32_t seed;       /* stochastic rounding seed */
@@ -157,6 +161,7 @@ typedef enum pskel_buffer {
 typedef struct pskel_grid_info {
     int32_t ndim, storage, elem_bytes, parity;
     size_t width, height, depth, pitch, bytes_per_buffer;
+    float store_offset;  /* the offset 16-bit storage holds u - store_offset against */
     void* current;   /* device pointer of the buffer holding the latest state */
     void* previous;  /* device pointer of the other ping-pong buffer */
     void* stream;    /* the grid's hipStream_t */
@@ -189,7 +194,10 @@ pskel_status_t pskel_grid_destroy(pskel_grid_t* grid);
 /* Copy a host array (tightly packed, Array2D/Array3D layout) into the INPUT or
  * OUTPUT buffer.  Uploading INPUT makes it the current state (parity 0).
  * host_dtype: the storage type (raw copy) or F32/F64 (converted, rounding to
- * nearest even for BF16/F16 storage). */
+ * nearest even for BF16/F16 storage).  With an automatic storage offset an
+ * INPUT upload takes F32/F64 data and first sets the offset to the mean of its
+ * finite values; OUTPUT uploads encode against the offset in force (upload
+ * INPUT first: 3D sweeps overwrite every cell of OUTPUT anyway). */
 pskel_status_t pskel_grid_upload(pskel_grid_t* grid, int32_t which, const void* host,
                                  int32_t host_dtype);
 /* Copy the current state (the result of the last run) into a host array. */
@@ -223,6 +231,23 @@ pskel_status_t pskel_run_iterative_host(const pskel_grid_desc_t* desc, const voi
                                         int32_t host_dtype, const pskel_stencil_desc_t* stencil,
                                         size_t iterations, size_t tile, size_t inner,
                                         const pskel_launch_cfg_t* cfg, pskel_run_stats_t* stats);
+
+/* Out-of-core launch search: the slab height (`tile`) and sweeps per pass
+ * (`inner`) for pskel_run_iterative_host.  Replaces the tiled branch of
+ * runIterativeAutoGPU (include/PSkelStencil.hpp:2014-2074), whose GA picks the
+ * tile and its sub-iterations by an analytic objective under a memory-fit
+ * constraint (:1883-1895, :1943-1957).  Candidates: slabs of 16 .. 1024
+ * rows/planes (at most half the grid) and 2, 4, 8 or 16 sweeps per pass whose
+ * device slots fit in `device_bytes` (0: half the free memory); each is timed
+ * through pskel_run_iterative_host on a sample of `host_in` (its first
+ * min(height, 4 * tile) rows/planes, one pass, copied: host_in is only read).
+ * *tile_out, *inner_out: the fastest per cell-update; ms_per_sweep (may be
+ * NULL): its time per sweep scaled to the whole grid (0 when the result comes
+ * from the process cache: a shape, stencil, iteration count and budget are
+ * searched once). */
+pskel_status_t pskel_tuner_host_search(const pskel_grid_desc_t* desc, const void* host_in, int32_t host_dtype,
+                                       const pskel_stencil_desc_t* stencil, size_t iterations, size_t device_bytes,
+                                       size_t* tile_out, size_t* inner_out, double* ms_per_sweep);
 
 /* Record one sweep launched outside the library on the grid's buffers (the
  * drop-in headers' device path for user stencil kernels with no descriptor):
@@ -272,17 +297,24 @@ pskel_status_t pskel_field_info(const pskel_field_t* field, void** device_ptr, s
 pskel_status_t pskel_grid_set_field(pskel_grid_t* grid, int32_t slot, pskel_field_t* field);
 
 /* ---- batched persistent sweeps: many uneven 2D grids advanced together by
- *      one persistent kernel per sweep (a fixed set of blocks walking a list
- *      of (grid, tile) work items), the sweep sequence captured in one
- *      hipGraph.  The reference runs one Stencil at a time with a launch and a
- *      host sync per sweep (include/PSkelStencil.hpp:1078-1082, 1790-1791).
+ *      persistent kernels (a fixed set of blocks walking a list of (grid,
+ *      tile) work items), the launch sequence captured in one hipGraph.  The
+ *      reference runs one Stencil at a time with a launch and a host sync per
+ *      sweep (include/PSkelStencil.hpp:1078-1082, 1790-1791).
  *      Grids: 2D, one device, Jacobi 5-point or 3x3 weighted (F32) or life
- *      (U8); cfgs[i].tile_y picks the rows per lane of grid i (1, 2, 4, 8). */
+ *      (U8).  cfgs[i].time_tile is grid i's temporal depth, the sweeps it fuses
+ *      per launch: 1 (or 0) or 3 for F32 (LDS temporal tiles), 1, 3, 5 or 7
+ *      for life (bit-packed tiles, mask range 1); odd, so every launch moves a
+ *      grid's state to its other buffer.  N sweeps run as N / k launches of k
+ *      and the rest as one odd launch or two (r - 1, then 1).  cfgs[i].tile_y:
+ *      at depth 1 the rows per lane (1, 2, 4, 8; default 4), at depth 3 in F32
+ *      the tile rows (16 or 32; default 16).  cfgs may be NULL (depth 1, 4 rows). */
 typedef struct pskel_batch pskel_batch_t;
 pskel_status_t pskel_batch_create(pskel_grid_t* const* grids, const pskel_stencil_desc_t* descs,
                                   const pskel_launch_cfg_t* cfgs, size_t n, pskel_batch_t** out);
 /* `iterations` sweeps of every grid; use_graph: replay a captured hipGraph
- * (captured on first use for this iteration count). */
+ * (captured on first use for this iteration count).  stats->launches: the
+ * launch steps (the longest grid's schedule). */
 pskel_status_t pskel_batch_run(pskel_batch_t* b, size_t iterations, int32_t use_graph,
                                pskel_run_stats_t* stats);
 pskel_status_t pskel_batch_info(const pskel_batch_t* b, int32_t* nitems, int32_t* nblocks);
@@ -314,8 +346,10 @@ pskel_status_t pskel_tuner_features(const pskel_tune_query_t* q, const pskel_ste
                                     const pskel_launch_cfg_t* c, float* feat);
 /* The candidate with the lowest predicted time.  model NULL: the model installed
  * with pskel_tuner_set_default_model, else the model shipped next to the
- * library (<libdir>/data/costmodel.bin, loaded on first use), else a fixed
- * heuristic per kernel family. */
+ * library (<libdir>/data/costmodel.bin, loaded on first use; the environment
+ * variable PSKEL_TUNER_MODEL names another file, or "none"), else a fixed
+ * heuristic per kernel family (the drop-in headers search by measurement,
+ * pskel_tuner_search, instead). */
 pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_query_t* q,
                                   const pskel_stencil_desc_t* d, pskel_launch_cfg_t* out);
 pskel_status_t pskel_tuner_set_default_model(pskel_mlp_t* model);
@@ -327,6 +361,33 @@ pskel_status_t pskel_tuner_heuristic(const pskel_tune_query_t* q, const pskel_st
  * reps runs after one warm-up; the grid state advances). */
 pskel_status_t pskel_tuner_measure(pskel_grid_t* g, const pskel_stencil_desc_t* d, size_t sweeps,
                                    const pskel_launch_cfg_t* c, int32_t reps, double* ms_per_sweep);
+
+/* 1 in *out when pskel_tuner_choose(NULL, ...) has a cost model (installed, or
+ * the file shipped next to the library, or the file named by the environment
+ * variable PSKEL_TUNER_MODEL; PSKEL_TUNER_MODEL=none: no model), else 0. */
+pskel_status_t pskel_tuner_has_model(int32_t* out);
+
+typedef enum pskel_search {
+    PSKEL_SEARCH_EXHAUSTIVE = 0, /* time every candidate */
+    PSKEL_SEARCH_GENETIC = 1     /* simple GA over candidate indices, each distinct candidate timed once */
+} pskel_search_t;
+
+/* Measured search over the candidates of grid g's shape, storage and stencil
+ * for `iterations` sweeps: the tuner's fallback when there is no cost model,
+ * and the yardstick of the model's regret.  Replaces the GAlib search of
+ * runIterativeAutoGPU (include/PSkelStencil.hpp:1865-2005), which scored an
+ * analytic objective and timed nothing.  Each candidate is timed with
+ * pskel_tuner_measure (`sweeps` sweeps, 0: min(iterations, 24); median of
+ * `reps` after a warm-up); GENETIC runs population 16 for 8 generations
+ * (binary genome, sigma-truncation fitness, one-point crossover 0.6, bit
+ * mutation 0.01) from a fixed seed.  The grid's state advances: upload again
+ * before the real run.  Results are kept for the process: the same shape,
+ * stencil, method and sweep count are searched once.  out: the fastest
+ * candidate; ms_per_sweep and measured (may be NULL): its time per sweep and
+ * the number of candidates timed by this call (0 when cached). */
+pskel_status_t pskel_tuner_search(pskel_grid_t* g, const pskel_stencil_desc_t* d, size_t iterations,
+                                  int32_t method, size_t sweeps, int32_t reps, pskel_launch_cfg_t* out,
+                                  double* ms_per_sweep, int32_t* measured);
 
 /* ---- cost model: a small MLP trained and evaluated on the device by the
  *      library's own kernels (forward, backward, Adam).  Inputs are

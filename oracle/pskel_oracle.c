@@ -451,8 +451,20 @@ static double sweep3d_f64(const plan3d* p, const double* in, double* out, size_t
 typedef float (*dec_fn)(uint16_t);
 typedef uint16_t (*enc_fn)(float);
 
-/* Reduced-precision storage: decode (+offset) -> float arithmetic in the f32
- * order -> subtract offset -> round to the storage format. */
+/* Reduced-precision storage of x = u - off.
+ *   scaled sum:   the stored values of the points (an out-of-range point of a
+ *                 HALO mask reads halo - off) are summed in mask order, then
+ *                 v = fma(coeff, sum, c0) with c0 = (coeff * n - 1) * off
+ *                 evaluated in double and rounded once to float: since
+ *                 coeff * sum(x_i + off) - off = coeff * sum(x_i) + c0, v is
+ *                 the new stored value, rounded to the storage format;
+ *   weighted sum: each point decodes to u = x + off, the f32 expression runs
+ *                 in mask order, and (result - off) is rounded to storage. */
+float oracle_scaled_sum_offset_term(float coeff, int npoints, float off)
+{
+    return (float)(((double)coeff * (double)npoints - 1.0) * (double)off);
+}
+
 static double sweep3d_r16(const plan3d* p, int store, float off, const uint16_t* in,
                           uint16_t* out, size_t W, size_t H, size_t D, int th, int want_res,
                           float halo, int rounding, uint32_t seed, uint64_t sweep)
@@ -462,6 +474,9 @@ static double sweep3d_r16(const plan3d* p, int store, float off, const uint16_t*
     const size_t r = p->r;
     const uint32_t key = oracle_sr_key(seed, sweep);
     const uint32_t cell_key = oracle_sr_cell_key(seed);
+    const int scaled = p->kind == ORACLE_KIND_SCALED_SUM;
+    const float c0 = oracle_scaled_sum_offset_term(p->cf, p->np, off);
+    const float hx = scaled ? halo - off : halo;
     double* rows = want_res ? (double*)calloc(H, sizeof(double)) : NULL;
 #pragma omp parallel for num_threads(th) schedule(static)
     for (size_t h = 0; h < H; ++h) {
@@ -480,19 +495,22 @@ static double sweep3d_r16(const plan3d* p, int store, float off, const uint16_t*
                         size_t hh = (size_t)((ptrdiff_t)h + p->off[3 * i]);
                         size_t ww = (size_t)((ptrdiff_t)w + p->off[3 * i + 1]);
                         size_t de = (size_t)((ptrdiff_t)dd + p->off[3 * i + 2]);
-                        float v = (ww < W && hh < H && de < D)
-                                      ? dec(in[(hh * W + ww) * D + de]) + off
-                                      : halo;
-                        if (p->kind == ORACLE_KIND_SCALED_SUM) acc = (i == 0) ? v : acc + v;
+                        float v = hx;
+                        if (ww < W && hh < H && de < D) {
+                            v = dec(in[(hh * W + ww) * D + de]);
+                            if (!scaled) v = v + off;
+                        }
+                        if (scaled) acc = (i == 0) ? v : acc + v;
                         else acc = (i == 0) ? p->wf[i] * v : acc + p->wf[i] * v;
                     }
-                    if (p->kind == ORACLE_KIND_SCALED_SUM) acc = p->cf * acc;
+                    /* the value to store: the new x (scaled) or u - off (weighted) */
+                    const float x = scaled ? fmaf(p->cf, acc, c0) : acc - off;
                     if (rounding == ORACLE_ROUND_STOCHASTIC) {
                         const uint32_t rb = oracle_sr_cell_bits(cell_key, w * D + dd) ^ plane_bits;
-                        sv = store == ORACLE_STORE_BF16 ? oracle_f32_to_bf16_sr(acc - off, rb)
-                                                        : oracle_f32_to_f16_sr(acc - off, rb);
+                        sv = store == ORACLE_STORE_BF16 ? oracle_f32_to_bf16_sr(x, rb)
+                                                        : oracle_f32_to_f16_sr(x, rb);
                     } else {
-                        sv = enc(acc - off);
+                        sv = enc(x);
                     }
                 }
                 if (want_res) {

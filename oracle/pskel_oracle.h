@@ -107,9 +107,17 @@ size_t oracle_run_iterative_3d_f32(const oracle_desc* d, float* input, float* ou
 size_t oracle_run_iterative_3d_f64(const oracle_desc* d, double* input, double* output,
                                    size_t width, size_t height, size_t depth, size_t iterations,
                                    int threads, size_t residual_every, double* residuals);
-/* Reduced-precision storage (16-bit words: bf16 or f16 bit patterns), float
- * arithmetic in the same order as the f32 runner, round-to-nearest-even on store.
- * Stored words hold (u - desc->store_offset). */
+/* Reduced-precision storage (16-bit words: bf16 or f16 bit patterns) of
+ * x = u - desc->store_offset, float arithmetic, round-to-nearest-even on store.
+ * A scaled sum adds the stored x of its points in mask order (a HALO point out
+ * of range reads halo_value - offset) and stores fma(coeff, sum, c0), with
+ * c0 = oracle_scaled_sum_offset_term(coeff, npoints, offset): the offset folded
+ * into one term, since coeff * sum(x_i + off) - off = coeff * sum(x_i) + c0.
+ * A weighted sum decodes u = x + offset, runs the f32 expression in mask order
+ * and stores (result - offset).  No reference counterpart: the reference has no
+ * reduced-precision path; this is the build's definition of one, pinned by
+ * error bounds against the f32 and f64 runners for any offset. */
+float oracle_scaled_sum_offset_term(float coeff, int npoints, float off);
 size_t oracle_run_iterative_3d_r16(const oracle_desc* d, int store, uint16_t* input,
                                    uint16_t* output, size_t width, size_t height, size_t depth,
                                    size_t iterations, int threads, size_t residual_every,

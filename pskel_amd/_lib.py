@@ -14,6 +14,7 @@ MAX_POINTS = 64
 
 # pskel_status_t
 OK = 0
+SEARCH_EXHAUSTIVE, SEARCH_GENETIC = 0, 1
 ERR_INVALID_ARGUMENT = 1
 ERR_OUT_OF_MEMORY = 2
 ERR_NO_DEVICE = 3
@@ -113,6 +114,7 @@ class GridInfo(C.Structure):
         ("depth", C.c_size_t),
         ("pitch", C.c_size_t),
         ("bytes_per_buffer", C.c_size_t),
+        ("store_offset", C.c_float),
         ("current", C.c_void_p),
         ("previous", C.c_void_p),
         ("stream", C.c_void_p),
@@ -204,6 +206,11 @@ def _declare(lib):
     _sig(lib, "pskel_tuner_heuristic", st, C.POINTER(TuneQuery), C.POINTER(StencilDesc), C.POINTER(LaunchCfg))
     _sig(lib, "pskel_tuner_set_default_model", st, P)
     _sig(lib, "pskel_tuner_measure", st, P, C.POINTER(StencilDesc), SZ, C.POINTER(LaunchCfg), I32, C.POINTER(D))
+    _sig(lib, "pskel_tuner_has_model", st, C.POINTER(I32))
+    _sig(lib, "pskel_tuner_host_search", st, C.POINTER(GridDesc), P, I32, C.POINTER(StencilDesc), SZ, SZ,
+         C.POINTER(SZ), C.POINTER(SZ), C.POINTER(D))
+    _sig(lib, "pskel_tuner_search", st, P, C.POINTER(StencilDesc), SZ, I32, SZ, I32, C.POINTER(LaunchCfg),
+         C.POINTER(D), C.POINTER(I32))
     _sig(lib, "pskel_mlp_create", st, C.POINTER(I32), I32, C.c_uint64, I32, C.POINTER(P))
     _sig(lib, "pskel_mlp_destroy", st, P)
     _sig(lib, "pskel_mlp_num_params", st, P, C.POINTER(SZ))

@@ -46,9 +46,11 @@ def _host_dtype(arr: np.ndarray, storage: int) -> int:
 
 
 class Grid:
-    """A device-resident grid pair (input/output ping-pong) on one GPU."""
+    """A device-resident grid pair (input/output ping-pong) on one GPU.
+    store_offset: what 16-bit storage holds u - offset against; "auto": the
+    mean of each INPUT upload (F32/F64 data)."""
 
-    def __init__(self, shape: Sequence[int], storage: str = "f32", store_offset: float = 0.0,
+    def __init__(self, shape: Sequence[int], storage: str = "f32", store_offset=0.0,
                  device: int = 0, rounding: str = "nearest", seed: int = 0):
         if len(shape) not in (2, 3):
             raise ValueError("shape is (height, width) or (height, width, depth)")
@@ -62,7 +64,7 @@ class Grid:
         gd.height = self.shape[0]
         gd.width = self.shape[1]
         gd.depth = self.shape[2] if self.ndim == 3 else 1
-        gd.store_offset = store_offset
+        gd.store_offset = float("nan") if store_offset == "auto" else float(store_offset)
         gd.device = device
         gd.rounding = {"nearest": L.ROUND_NEAREST, "stochastic": L.ROUND_STOCHASTIC}[rounding]
         gd.seed = int(seed) & 0xFFFFFFFF
@@ -205,11 +207,13 @@ class Field:
 
 
 class Batch:
-    """Many 2D grids swept together by one persistent kernel per sweep
+    """Many 2D grids swept together by one persistent kernel per launch
     (pskel_batch_t).  grids: list of Grid; stencils: one per grid; rows: rows
-    per lane per grid (1, 2, 4 or 8)."""
+    per lane per grid (1, 2, 4 or 8) at depth 1, tile rows at depth > 1 (f32:
+    16 or 32; life: 32); depths: sweeps fused per launch per grid (odd: 1 or 3
+    for f32, 1, 3, 5 or 7 for life)."""
 
-    def __init__(self, grids, stencils, rows=None):
+    def __init__(self, grids, stencils, rows=None, depths=None):
         n = len(grids)
         self.grids = list(grids)
         hs = (C.c_void_p * n)(*[g._h for g in grids])
@@ -217,7 +221,9 @@ class Batch:
         cs = (L.LaunchCfg * n)()
         for i in range(n):
             cs[i].variant = L.VARIANT_TUNED
-            cs[i].tile_y = int(rows[i]) if rows is not None else 4
+            k = int(depths[i]) if depths is not None else 1
+            cs[i].time_tile = k
+            cs[i].tile_y = int(rows[i]) if rows is not None and rows[i] else (4 if k == 1 else 0)
         h = C.c_void_p()
         L.check(L.lib().pskel_batch_create(hs, ds, cs, n, C.byref(h)))
         self._h = h

@@ -32,7 +32,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -98,6 +101,23 @@ size_t last_reader(const std::vector<Slab>& v, size_t h)
     size_t i = 0;
     while (i + 1 < v.size() && v[i + 1].s0 < h) ++i;
     return i;
+}
+
+// Device bytes the pipeline holds: lag + 2 slots of the widest staged slab,
+// each with 2 buffers (3 in 2D, whose temporal tiles of even depth use a spare).
+size_t pipeline_device_bytes(int ndim, size_t W, size_t H, size_t D, int elem, size_t tile, size_t halo)
+{
+    const std::vector<Slab> widest = make_slabs(H, tile, halo);
+    size_t max_rows = 0, lag = 0;
+    for (size_t j = 0; j < widest.size(); ++j) {
+        max_rows = std::max(max_rows, widest[j].s1 - widest[j].s0);
+        lag = std::max(lag, last_reader(widest, widest[j].h1) - j);
+    }
+    const size_t nslots = std::min(widest.size(), lag + 2);
+    const size_t row_bytes = (ndim == 2 ? W : D) * (size_t)elem;
+    const size_t pitch_bytes = (row_bytes + 1023) / 1024 * 1024;
+    const size_t plane_rows = ndim == 2 ? 1 : W;
+    return nslots * pitch_bytes * plane_rows * max_rows * (ndim == 2 ? 3 : 2);
 }
 
 // Everything one call owns; the destructor drains the streams before it frees.
@@ -231,6 +251,8 @@ extern "C" pskel_status_t pskel_run_iterative_host(const pskel_grid_desc_t* gd, 
     if (elem == 0) return fail(PSKEL_ERR_UNSUPPORTED, "storage type");
     if (ndim == 2 && elem == 2) return fail(PSKEL_ERR_UNSUPPORTED, "16-bit storage is implemented for 3D grids");
     if (ndim == 3 && elem == 1) return fail(PSKEL_ERR_UNSUPPORTED, "u8 storage is implemented for 2D grids");
+    if (std::isnan(gd->store_offset))
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "host arrays hold storage words: give their storage offset");
     if (host_dtype != gd->storage)
         return fail(PSKEL_ERR_INVALID_ARGUMENT, "host-staged sweeps take host arrays in the storage type");
     if (tile == 0 || inner == 0) return fail(PSKEL_ERR_INVALID_ARGUMENT, "tile and inner must be positive");
@@ -405,5 +427,101 @@ extern "C" pskel_status_t pskel_run_iterative_host(const pskel_grid_desc_t* gd, 
         stats->elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         stats->launches = launches;
     }
+    return PSKEL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Out-of-core launch search: runIterativeAutoGPU's tiled branch
+// (include/PSkelStencil.hpp:2014-2074) picks the tile and its sub-iterations
+// with a GA over an analytic objective under a memory-fit constraint
+// (:1883-1895, :1943-1957).  Here each candidate (slab height, sweeps per
+// pass) that fits the device budget is timed through the pipeline itself on a
+// sample of the grid: its first min(H, 4 * slab) rows or planes, one pass.
+// ---------------------------------------------------------------------------
+namespace {
+struct HostSearchKey {
+    std::vector<unsigned char> bytes;
+    bool operator<(const HostSearchKey& o) const { return bytes < o.bytes; }
+};
+std::mutex g_host_search_mutex;
+std::map<HostSearchKey, std::pair<size_t, size_t>> g_host_search_cache;
+}  // namespace
+
+extern "C" pskel_status_t pskel_tuner_host_search(const pskel_grid_desc_t* gd, const void* host_in, int32_t host_dtype,
+                                                  const pskel_stencil_desc_t* d, size_t iterations,
+                                                  size_t device_bytes, size_t* tile_out, size_t* inner_out,
+                                                  double* ms_per_sweep)
+{
+    if (!gd || !host_in || !d || !tile_out || !inner_out) return fail(PSKEL_ERR_INVALID_ARGUMENT, "NULL argument");
+    const int ndim = gd->ndim;
+    if (ndim != 2 && ndim != 3) return fail(PSKEL_ERR_INVALID_ARGUMENT, "ndim must be 2 or 3");
+    const int elem = storage_bytes(gd->storage);
+    if (elem == 0 || host_dtype != gd->storage)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "host-staged sweeps take host arrays in the storage type");
+    if (d->npoints < 1 || d->npoints > PSKEL_MAX_POINTS) return fail(PSKEL_ERR_INVALID_ARGUMENT, "npoints");
+    const size_t W = gd->width, H = gd->height, D = ndim == 3 ? gd->depth : 1;
+    if (W == 0 || H == 0 || D == 0) return fail(PSKEL_ERR_INVALID_ARGUMENT, "empty grid");
+    if (iterations == 0) iterations = 1;
+    if (device_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        PSKEL_HIP(hipSetDevice(gd->device));
+        PSKEL_HIP(hipMemGetInfo(&fr, &tot));
+        device_bytes = fr / 2;
+    }
+    const size_t radius = (size_t)desc_radius(*d, ndim);
+    const size_t plane_cells = ndim == 2 ? W : W * D;
+    const size_t plane_bytes = plane_cells * (size_t)elem;
+    HostSearchKey key;
+    auto put = [&key](const void* p, size_t n) {
+        key.bytes.insert(key.bytes.end(), (const unsigned char*)p, (const unsigned char*)p + n);
+    };
+    put(gd, sizeof(*gd));
+    put(d, sizeof(*d));
+    put(&iterations, sizeof(iterations));
+    put(&device_bytes, sizeof(device_bytes));
+    {
+        std::lock_guard<std::mutex> lock(g_host_search_mutex);
+        auto it = g_host_search_cache.find(key);
+        if (it != g_host_search_cache.end()) {
+            *tile_out = it->second.first;
+            *inner_out = it->second.second;
+            if (ms_per_sweep) *ms_per_sweep = 0.0;
+            return PSKEL_OK;
+        }
+    }
+    double best = -1.0;
+    size_t bt = 0, bi = 0;
+    pskel_launch_cfg_t cfg{};
+    cfg.variant = PSKEL_VARIANT_TUNED;
+    for (size_t tile : {16, 32, 64, 128, 256, 512, 1024}) {
+        if (tile > std::max<size_t>(H / 2, 1) && tile != 16) continue;
+        for (size_t inner : {2, 4, 8, 16}) {
+            if (inner > iterations && inner != 2) continue;
+            const size_t in_eff = std::min(inner, iterations);
+            if (pipeline_device_bytes(ndim, W, H, D, elem, tile, radius * in_eff) > device_bytes) continue;
+            const size_t hs = std::min(H, 4 * tile);
+            pskel_grid_desc_t gs = *gd;
+            gs.height = hs;
+            std::vector<unsigned char> a((const unsigned char*)host_in, (const unsigned char*)host_in + hs * plane_bytes);
+            std::vector<unsigned char> b(a);
+            pskel_run_stats_t st{};
+            const pskel_status_t s = pskel_run_iterative_host(&gs, a.data(), b.data(), host_dtype, d, in_eff, tile,
+                                                              in_eff, &cfg, &st);
+            if (s != PSKEL_OK) continue;  // the pipeline cannot run this shape here
+            const double per_cu = st.elapsed_ms / ((double)hs * (double)plane_cells * (double)in_eff);
+            if (best < 0.0 || per_cu < best) {
+                best = per_cu;
+                bt = tile;
+                bi = in_eff;
+            }
+        }
+    }
+    if (best < 0.0) return fail(PSKEL_ERR_OUT_OF_MEMORY, "no slab fits the device budget");
+    set_error("");
+    *tile_out = bt;
+    *inner_out = bi;
+    if (ms_per_sweep) *ms_per_sweep = best * (double)H * (double)plane_cells;
+    std::lock_guard<std::mutex> lock(g_host_search_mutex);
+    g_host_search_cache[key] = {bt, bi};
     return PSKEL_OK;
 }

@@ -58,6 +58,9 @@ __host__ __device__ __forceinline__ uint32_t sr_cell_bits(uint32_t cell_key, uin
 // ---------------------------------------------------------------------------
 // Storage formats.  Arithmetic is always f32; 16-bit formats store (u - off),
 // rounded to nearest even (enc) or stochastically (enc_sr) on the way out.
+// dec / enc / enc_sr add and subtract the offset (weighted sums); dec_raw /
+// enc_raw / enc_sr_raw work on the stored values themselves (scaled sums,
+// whose offset term is folded into one fused multiply-add).
 // Stochastic rounding is gfx950's v_cvt_sr_{bf16,f16}_f32: the top 16 (bf16)
 // or 13 (f16) bits of the random word are added below the kept bits of the
 // magnitude, which is then truncated, so it rounds up with probability equal
@@ -75,6 +78,8 @@ struct Stor<PSKEL_DTYPE_F32> {
     __device__ __forceinline__ static float dec_raw(T v) { return v; }
     __device__ __forceinline__ static T enc(float v, float) { return v; }
     __device__ __forceinline__ static T enc_sr(float v, float, uint32_t) { return v; }
+    __device__ __forceinline__ static T enc_raw(float v) { return v; }
+    __device__ __forceinline__ static T enc_sr_raw(float v, uint32_t) { return v; }
 };
 
 template <>
@@ -88,15 +93,32 @@ struct Stor<PSKEL_DTYPE_BF16> {
         const __bf16 b = (__bf16)(v - off);
         return __builtin_bit_cast(T, b);
     }
-    __device__ __forceinline__ static T enc_sr(float v, float off, uint32_t r)
+    __device__ __forceinline__ static T enc_sr(float v, float off, uint32_t r) { return enc_sr_raw(v - off, r); }
+    __device__ __forceinline__ static T enc_raw(float v)
+    {
+        const __bf16 b = (__bf16)v;
+        return __builtin_bit_cast(T, b);
+    }
+    __device__ __forceinline__ static T enc_sr_raw(float v, uint32_t r)
     {
         // the converted half only: the other half is left unspecified (no
         // register initialisation)
         bf16x2_t o = __builtin_nondeterministic_value(o);
-        o = __builtin_amdgcn_cvt_sr_bf16_f32(o, v - off, r, false);
+        o = __builtin_amdgcn_cvt_sr_bf16_f32(o, v, r, false);
         return __builtin_bit_cast(T, o[0]);
     }
 };
+
+// The value as rounded to f32, hidden from the optimiser: gfx9 has a
+// mixed-precision FMA that rounds an f32 product-sum straight to f16
+// (v_fma_mix*_f16), one rounding where the storage expression rounds twice
+// (to f32, then to storage).  An empty asm with the value in a VGPR keeps the
+// conversion on the f32 result; it emits no instruction.
+__device__ __forceinline__ float as_rounded_f32(float x)
+{
+    asm("" : "+v"(x));
+    return x;
+}
 
 template <>
 struct Stor<PSKEL_DTYPE_F16> {
@@ -105,12 +127,14 @@ struct Stor<PSKEL_DTYPE_F16> {
     __device__ __forceinline__ static float dec(T v, float off) { return dec_raw(v) + off; }
     __device__ __forceinline__ static T enc(float v, float off)
     {
-        return __half_as_ushort(__float2half_rn(v - off));
+        return __half_as_ushort(__float2half_rn(as_rounded_f32(v - off)));
     }
-    __device__ __forceinline__ static T enc_sr(float v, float off, uint32_t r)
+    __device__ __forceinline__ static T enc_sr(float v, float off, uint32_t r) { return enc_sr_raw(v - off, r); }
+    __device__ __forceinline__ static T enc_raw(float v) { return __half_as_ushort(__float2half_rn(as_rounded_f32(v))); }
+    __device__ __forceinline__ static T enc_sr_raw(float v, uint32_t r)
     {
         f16x2_t o = __builtin_nondeterministic_value(o);
-        o = __builtin_amdgcn_cvt_sr_f16_f32(o, v - off, r, false);
+        o = __builtin_amdgcn_cvt_sr_f16_f32(o, v, r, false);
         return __builtin_bit_cast(T, o[0]);
     }
 };

@@ -9,7 +9,9 @@
 // no face tests.  Every cell evaluates
 //     coeff * (((((h- + h+) + w-) + w+) + d-) + d+)
 // in f32 with no contraction, the order of the Jacobi mask
-// (-1,0,0) (1,0,0) (0,-1,0) (0,1,0) (0,0,-1) (0,0,1).
+// (-1,0,0) (1,0,0) (0,-1,0) (0,1,0) (0,0,-1) (0,0,1).  16-bit storage sums
+// the stored values x = u - off in the same order and stores
+// fma(coeff, sum, c0), with c0 = (6 * coeff - 1) * off (scaled_sum_offset_term).
 //
 // Two forms of the register queue along h, picked by element size (both
 // measured on MI355X, 512^3):
@@ -95,7 +97,13 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
     const int W = s.W, H = s.H, D = s.D, r = s.radius;
     const int pitch = (int)s.pitch;
     const size_t plane = (size_t)W * s.pitch;
-    const float off = s.store_offset;
+    // 16-bit storage: the stored values (u - off) are summed as they are and
+    // the offset term folds into one fused multiply-add (scaled_sum_offset_term)
+    const float c0 = s.c0;
+    auto encode = [&](float acc, uint32_t rw) __attribute__((always_inline)) {
+        const float v = ST == PSKEL_DTYPE_F32 ? coeff * acc : __builtin_fmaf(coeff, acc, c0);
+        return SR ? Tr::enc_sr_raw(v, rw) : Tr::enc_raw(v);
+    };
 
     const int d0 = td * TD, w0 = tw * TW;
     const int d = d0 + lane * VD;
@@ -195,7 +203,7 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
 #pragma unroll
         for (int i = 0; i < RW; ++i)
 #pragma unroll
-            for (int j = 0; j < VD; ++j) v[i][j] = Tr::dec(raw[i][j], off);
+            for (int j = 0; j < VD; ++j) v[i][j] = Tr::dec_raw(raw[i][j]);
     };
 
     // pl[0] = plane h0-1 (only read at h0 > 0), pl[1] = plane h0, slots 0/1 = planes h0, h0+1
@@ -244,13 +252,13 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
         for (int i = 0; i < RW; ++i) {
 #pragma unroll
             for (int j = 0; j < VD; ++j) L[1 + wave * RW + i][VD + lane * VD + j] = cur[i][j];
-            if (hal_dlo) L[1 + wave * RW + i][VD - 1] = Tr::dec(qd[CUR][i], off);
-            if (hal_dhi) L[1 + wave * RW + i][VD + TD] = Tr::dec(qd[CUR][i], off);
+            if (hal_dlo) L[1 + wave * RW + i][VD - 1] = Tr::dec_raw(qd[CUR][i]);
+            if (hal_dhi) L[1 + wave * RW + i][VD + TD] = Tr::dec_raw(qd[CUR][i]);
         }
         if (hal_wlo || hal_whi) {
             const int row = hal_wlo ? 0 : TW + 1;
 #pragma unroll
-            for (int j = 0; j < VD; ++j) L[row][VD + lane * VD + j] = Tr::dec(qw[CUR][j], off);
+            for (int j = 0; j < VD; ++j) L[row][VD + lane * VD + j] = Tr::dec_raw(qw[CUR][j]);
         }
         __syncthreads();
 
@@ -273,7 +281,7 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
                     acc = acc + wp;
                     acc = acc + dm;
                     acc = acc + dp;
-                    o[j] = SR ? Tr::enc_sr(coeff * acc, off, ra[i][j] ^ rbh) : Tr::enc(coeff * acc, off);
+                    o[j] = encode(acc, ra[i][j] ^ rbh);
                 }
             } else {
                 const bool wface = !hint || !row_int[i];
@@ -292,7 +300,7 @@ __global__ __launch_bounds__(256) void k3d_march(Sweep3D s, float coeff, int hse
                         acc = acc + wp;
                         acc = acc + dm;
                         acc = acc + dp;
-                        o[j] = SR ? Tr::enc_sr(coeff * acc, off, ra[i][j] ^ rbh) : Tr::enc(coeff * acc, off);
+                        o[j] = encode(acc, ra[i][j] ^ rbh);
                     }
                 }
             }

@@ -198,34 +198,30 @@ struct LaneWords {
             return __half2float(__ushort_as_half((uint16_t)((j & 1) ? (w[j >> 1] >> 16) : w[j >> 1])));
         }
     }
-    // cell j's value
-    __device__ __forceinline__ static float cell(const W (&w)[N], int j, float off)
-    {
-        if constexpr (ST == PSKEL_DTYPE_F32) return w[j];
-        else return raw(w, j) + off;
-    }
-    // v - off rounded to storage, to nearest even (as Stor<ST>::enc)
-    __device__ __forceinline__ static void put(W (&w)[N], const float (&v)[VD], float off)
+    // v rounded to storage, to nearest even (as Stor<ST>::enc_raw)
+    __device__ __forceinline__ static void put(W (&w)[N], const float (&v)[VD])
     {
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             if constexpr (ST == PSKEL_DTYPE_F32) {
                 w[k] = v[k];
             } else {
-                const f32x2_t f = {v[2 * k] - off, v[2 * k + 1] - off};
+                // f16: each value rounded to f32 first (as_rounded_f32)
+                const f32x2_t f = ST == PSKEL_DTYPE_F16 ? f32x2_t{as_rounded_f32(v[2 * k]), as_rounded_f32(v[2 * k + 1])}
+                                                        : f32x2_t{v[2 * k], v[2 * k + 1]};
                 if constexpr (ST == PSKEL_DTYPE_BF16) w[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2_t));
                 else w[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, f16x2_t));
             }
         }
     }
-    // v - off rounded stochastically with the random word r[j] of each cell
-    // (as Stor<ST>::enc_sr); 16-bit only
-    __device__ __forceinline__ static void put_sr(W (&w)[N], const float (&v)[VD], float off, const uint32_t (&r)[VD])
+    // v rounded stochastically with the random word r[j] of each cell (as
+    // Stor<ST>::enc_sr_raw); 16-bit only
+    __device__ __forceinline__ static void put_sr(W (&w)[N], const float (&v)[VD], const uint32_t (&r)[VD])
     {
         static_assert(ST != PSKEL_DTYPE_F32, "stochastic rounding is for 16-bit storage");
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            const float a = v[2 * k] - off, b = v[2 * k + 1] - off;
+            const float a = v[2 * k], b = v[2 * k + 1];
             // the first conversion replaces the low half of a's own register,
             // the second the high half
             if constexpr (ST == PSKEL_DTYPE_BF16) {
@@ -250,6 +246,25 @@ struct LaneWords {
     // the mask of cell j within its word
     static constexpr uint32_t cell_mask(int j) { return CPW == 1 ? ~0u : 0xffffu << (16 * (j % CPW)); }
 };
+
+// Neighbour exchange along d across lanes: lane l receives lane l-1's value
+// (wave_shr:1) or lane l+1's (wave_shl:1); the lane with no source (lane 0,
+// respectively 63) keeps `edge`, the neighbouring wave's cell from LDS.  DPP
+// controls of the row-shift family on wave64 GFX9 parts (gfx950 included).
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppWaveShl1 = 0x130;
+
+__device__ __forceinline__ float from_lower_lane(float v, float edge)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), kDppWaveShr1, 0xf,
+                                                      0xf, false));
+}
+
+__device__ __forceinline__ float from_upper_lane(float v, float edge)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), kDppWaveShl1, 0xf,
+                                                      0xf, false));
+}
 
 template <int ST, int VD, int RW, int NWD, int NWW, int PF, bool RES, bool SR>
 __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float coeff, int hseg, int ntd, int ntw)
@@ -280,7 +295,9 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
     const int W = s.W, H = s.H, D = s.D, r = s.radius;
     const int pitch = (int)s.pitch;
     const size_t plane = (size_t)W * s.pitch;
-    const float off = s.store_offset;
+    // 16-bit storage: the stored values x = u - off are summed as they are and
+    // the result is fma(coeff, sum, c0) (scaled_sum_offset_term)
+    const float c0 = s.c0;
     // stochastic rounding: the keys of the launch's two sweeps
     const uint32_t key1 = SR ? sweep_key(s.seed, s.sweep) : 0u;
     const uint32_t key2 = SR ? sweep_key(s.seed, s.sweep + 1) : 0u;
@@ -320,6 +337,10 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
             row_int[i] = w >= r && w < W - r;
             row_out[i] = dout && w < W && w >= ow0 && w < ow1;
         }
+        // every cell of this wave interior along w and d (wave-uniform): on an
+        // interior plane its results need no face merge
+        const int dw0 = ed0 + wd * kWave * VD;
+        const bool wave_int = dw0 >= r && dw0 + kWave * VD <= D - r && wr >= r && wr + RW <= W - r;
         // the halo row this row of waves loads (top: the row above the tile, bottom: the row below)
         const bool top = ww == 0, bottom = ww == NWW - 1;
         const int hrow = top ? ew0 - 1 : ew0 + TW;
@@ -350,10 +371,11 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         }
 
         constexpr int NQ = PF + 3;  // ring slots: planes h-1 .. h+PF+1
-        Word in[NQ][RW][NV];           // input planes (storage words)
-        Word hal[NQ][NV];              // input planes of the halo row
-        Word s1[NQ][RW][NV];           // first-sweep planes h-2, h-1, h (the rest unused), storage words
-
+        Word in[NQ][RW][NV];        // input planes as loaded (storage words)
+        Word hal[NQ][NV];           // input planes of the halo row
+        float fin[NQ][RW][VD];      // input planes h-1, h, h+1, decoded once each
+        Word s1[NQ][RW][NV];        // first-sweep planes, storage words
+        float fs1[NQ][RW][VD];      // first-sweep planes h-2, h-1, h, decoded once each
 
         auto load = [&](int p, Word (&v)[RW][NV], Word (&hv)[NV]) {
             const bool ok = p >= hlo && p < hhi;  // uniform across the block
@@ -366,7 +388,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
             for (int i = 0; i < RW; ++i)
 #pragma unroll
-                for (int j = 0; j < VD; ++j) f[i][j] = LW::cell(w[i], j, off);
+                for (int j = 0; j < VD; ++j) f[i][j] = LW::raw(w[i], j);
         };
         // a plane's boundary rows and edge cells into LDS
         auto stage = [&](LV (*L)[NL], float (*E)[NWD][RW][2], const float (&v)[RW][VD]) {
@@ -386,7 +408,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         // With HALO, L carries the halo rows (the input plane); without, the
         // tile's edge rows take a stand-in neighbour (those cells are never output).
         // out: storage words; a face keeps its word `midw`, any other cell rounds
-        // coeff * sum (stochastically: with the word ra ^ B(sweep, plane), where
+        // its result (stochastically: with the word ra ^ B(sweep, plane), where
         // `key` is the sweep's key)
         auto sweep = [&](auto halo, int p, const float (&lo)[RW][VD], const float (&mid)[RW][VD],
                          const float (&hi)[RW][VD], const Word (&midw)[RW][NV], const LV (*L)[NL],
@@ -412,12 +434,8 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
                     lds_get<VD>(&L[slot][col], wp);
                 }
                 // cells d-1 and d+VD: the neighbouring lane's, or across the wave edge
-                float left = __shfl_up(mid[i][VD - 1], 1, kWave);
-                float right = __shfl_down(mid[i][0], 1, kWave);
-                const float eleft = E[ww][wd > 0 ? wd - 1 : 0][i][1];
-                const float eright = E[ww][wd < NWD - 1 ? wd + 1 : NWD - 1][i][0];
-                if (lane == 0) left = eleft;
-                if (lane == kWave - 1) right = eright;
+                const float left = from_lower_lane(mid[i][VD - 1], E[ww][wd > 0 ? wd - 1 : 0][i][1]);
+                const float right = from_upper_lane(mid[i][0], E[ww][wd < NWD - 1 ? wd + 1 : NWD - 1][i][0]);
                 float val[VD];
 #pragma unroll
                 for (int j = 0; j < VD; ++j) {
@@ -428,29 +446,38 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
                     acc = acc + wp[j];
                     acc = acc + dm;
                     acc = acc + dp;
-                    val[j] = coeff * acc;
+                    if constexpr (ST == PSKEL_DTYPE_F32) val[j] = coeff * acc;
+                    else val[j] = __builtin_fmaf(coeff, acc, c0);
                 }
-                Word nw[NV];
                 if constexpr (SR) {
                     uint32_t rw[VD];
 #pragma unroll
                     for (int j = 0; j < VD; ++j) rw[j] = ra[i][j] ^ rb;
-                    LW::put_sr(nw, val, off, rw);
+                    LW::put_sr(out[i], val, rw);
                 } else {
-                    LW::put(nw, val, off);
+                    LW::put(out[i], val);
                 }
-                // faces keep their word: the whole row on a face plane or row
-                // (uniform), else the lane's d-face cells
-                const uint32_t frow = (!pint || !row_int[i]) ? ~0u : 0u;
+            }
+            // faces keep their word: the whole row on a face plane or row
+            // (uniform), else the lane's d-face cells.  A wave with no face
+            // cell on an interior plane skips this (a uniform branch).
+            if (!(pint && wave_int)) {
 #pragma unroll
-                for (int k = 0; k < NV; ++k) out[i][k] = LW::merge(frow | dmask[k], midw[i][k], nw[k]);
+                for (int i = 0; i < RW; ++i) {
+                    const uint32_t frow = (!pint || !row_int[i]) ? ~0u : 0u;
+#pragma unroll
+                    for (int k = 0; k < NV; ++k) out[i][k] = LW::merge(frow | dmask[k], midw[i][k], out[i][k]);
+                }
             }
         };
 
 #pragma unroll
-        for (int i = 0; i < RW; ++i)
+        for (int i = 0; i < RW; ++i) {
 #pragma unroll
             for (int k = 0; k < NV; ++k) s1[0][i][k] = s1[1][i][k] = Word(0);
+#pragma unroll
+            for (int j = 0; j < VD; ++j) fs1[0][i][j] = fs1[1][i][j] = 0.f;
+        }
         // prologue: each load followed by discarded stores, the order of the
         // steady state, so the wait counts the compiler derives at the loop head
         // hold on every path into it
@@ -463,6 +490,9 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
                 for (int i = 0; i < RW; ++i) buf_store<Word, NV>(none, kNoAccess, s1[0][i]);
             }
         }
+        // planes ha-1 and ha, decoded (the steady state decodes plane h+1)
+        decode(in[0], fin[0]);
+        decode(in[1], fin[1]);
 
         int b = 0;
         // step h at ring phase S: input slot (S + k) % NQ holds plane h - 1 + k,
@@ -472,36 +502,33 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
             constexpr int I0 = S % NQ, I1 = (S + 1) % NQ, I2 = (S + 2) % NQ, IF = (S + NQ - 1) % NQ;
             load(h + PF + 1, in[IF], hal[IF]);
             const int p2 = h - 1;
-            float fm[RW][VD], gm[RW][VD];  // input plane h and s1(h-1), decoded
-            decode(in[I1], fm);
-            decode(s1[I1], gm);
-            stage(lin[b], lein[b], fm);
+            // plane h and s1(h-1) into LDS
+            stage(lin[b], lein[b], fin[I1]);
             if (top || bottom) {
                 float fh[VD];
 #pragma unroll
-                for (int j = 0; j < VD; ++j) fh[j] = LW::cell(hal[I1], j, off);
+                for (int j = 0; j < VD; ++j) fh[j] = LW::raw(hal[I1], j);
                 lds_put<VD>(&lin[b][hslot][col], fh);
             }
-            stage(ls1[b], les1[b], gm);
+            stage(ls1[b], les1[b], fs1[I1]);
             __syncthreads();
+            decode(in[I2], fin[I2]);  // plane h+1
             if (h < hb) {
-                float flo[RW][VD], fhi[RW][VD];
-                decode(in[I0], flo);
-                decode(in[I2], fhi);
-                sweep(std::true_type{}, h, flo, fm, fhi, in[I1], lin[b], lein[b], key1, s1[I2]);
+                sweep(std::true_type{}, h, fin[I0], fin[I1], fin[I2], in[I1], lin[b], lein[b], key1, s1[I2]);
+                decode(s1[I2], fs1[I2]);
             } else {
 #pragma unroll
-                for (int i = 0; i < RW; ++i)
+                for (int i = 0; i < RW; ++i) {
 #pragma unroll
                     for (int k = 0; k < NV; ++k) s1[I2][i][k] = Word(0);
+#pragma unroll
+                    for (int j = 0; j < VD; ++j) fs1[I2][i][j] = 0.f;
+                }
             }
             const bool out2 = p2 >= h0 && p2 < h1;  // uniform across the block
             Word o[RW][NV];
             if (out2) {
-                float glo[RW][VD], ghi[RW][VD];
-                decode(s1[I0], glo);
-                decode(s1[I2], ghi);
-                sweep(std::false_type{}, p2, glo, gm, ghi, s1[I1], ls1[b], les1[b], key2, o);
+                sweep(std::false_type{}, p2, fs1[I0], fs1[I1], fs1[I2], s1[I1], ls1[b], les1[b], key2, o);
                 if constexpr (RES) {
 #pragma unroll
                     for (int i = 0; i < RW; ++i) {
@@ -509,7 +536,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
                         for (int j = 0; j < VD; ++j) {
                             if (d + j < D) {
-                                const double dv = (double)LW::raw(o[i], j) - (double)LW::raw(s1[I1][i], j);
+                                const double dv = (double)LW::raw(o[i], j) - (double)fs1[I1][i][j];
                                 rr += dv * dv;
                             }
                         }

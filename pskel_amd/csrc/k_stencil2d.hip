@@ -95,108 +95,15 @@ hipError_t tuned_dispatch(const Sweep2D& s, const Points& p, int ry, int nwy, hi
 }
 
 // ---------------------------------------------------------------------------
-// temporal tile: core CX x CY cells, loaded region (CX + 8) x (CY + 2K) with a
-// 4-column aligned halo (K <= 4), K sweeps in LDS ping-pong.
-// Cells outside the interior are never computed: at level j an LDS buffer holds
-// the ring of the global buffer that level j lives in (level parity alternates
-// between src and ring_other), so halo reads near the grid edge see exactly what
-// a single sweep would see.
+// temporal tile: K sweeps per launch in LDS (the body is ttile_body in
+// k_tiles2d.hpp, shared with the batched kernel)
 // ---------------------------------------------------------------------------
 template <int PAT, int K, int CX, int CY>
 __global__ __launch_bounds__(256) void k2d_ttile(Sweep2D s, Points p)
 {
-    constexpr int LX = CX + 8;
-    constexpr int LY = CY + 2 * K;
-    constexpr int NT = 256;
-    __shared__ float A[LY * LX];
-    __shared__ float B[LY * LX];
-    const int tid = threadIdx.x;
-    const int w0 = blockIdx.x * CX;            // core origin
-    const int h0 = blockIdx.y * CY;
-    const int gx0 = w0 - 4, gy0 = h0 - K;      // loaded region origin
-    const int H = s.H, Wd = s.W, r = s.radius;
-    const size_t pitch = s.pitch;
-    const float* __restrict__ src = (const float*)s.src;
-    const float* __restrict__ oth = (const float*)s.ring_other;
-
-    // load: float4 granules, LX/4 per row
-    constexpr int G = LX / 4;
-    for (int q = tid; q < G * LY; q += NT) {
-        const int y = q / G, x4 = (q - y * G) * 4;
-        const int gh = gy0 + y, gw = gx0 + x4;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
-        if (gh >= 0 && gh < H && gw >= 0 && gw < (int)pitch) {
-            a = *reinterpret_cast<const float4*>(src + (size_t)gh * pitch + gw);
-            const bool ring_row = gh < r || gh >= H - r;
-            const bool ring_col = gw < r || gw + 3 >= Wd - r;
-            if (ring_row || ring_col) b = *reinterpret_cast<const float4*>(oth + (size_t)gh * pitch + gw);
-        }
-        *reinterpret_cast<float4*>(&A[y * LX + x4]) = a;
-        *reinterpret_cast<float4*>(&B[y * LX + x4]) = b;
-    }
-    __syncthreads();
-
-    // Level j: each thread walks one column x of the level's region down one
-    // of NS strips of rows, keeping the row above and its own row (x-1, x, x+1)
-    // in registers, so a cell costs three LDS reads (the row below).  Sums keep
-    // the mask order.
-    constexpr int NS = NT / 128;  // strips per column
-    static_assert(LX <= 128, "one thread per column of the loaded region");
-    const int col = tid % 128, strip = tid / 128;
-#pragma unroll
-    for (int j = 1; j <= K; ++j) {
-        const float* __restrict__ P = (j & 1) ? A : B;
-        float* __restrict__ Q = (j & 1) ? B : A;
-        const int xlo = 4 - (K - j), xhi = 4 + CX + (K - j);
-        const int ylo = j, yhi = LY - j;
-        const int nx = xhi - xlo, ny = yhi - ylo;
-        const int rows = (ny + NS - 1) / NS;
-        if (col < nx) {
-            const int x = xlo + col;
-            const int gw = gx0 + x;
-            const bool col_int = gw >= r && gw < Wd - r;
-            const int ya = ylo + strip * rows, yb = min(ya + rows, yhi);
-            const float* c = P + (ya - 1) * LX + x;
-            float um = c[-1], u0 = c[0], up = c[1];                 // row y - 1
-            float cm = c[LX - 1], c0 = c[LX], cp = c[LX + 1];       // row y
-#pragma unroll 2
-            for (int y = ya; y < yb; ++y) {
-                c += LX;
-                const float dm = c[LX - 1], d0 = c[LX], dp = c[LX + 1];  // row y + 1
-                float acc;
-                if constexpr (PAT == PAT_J5) {
-                    acc = u0 + d0;
-                    acc = acc + cm;
-                    acc = acc + cp;
-                    acc = p.coeff * acc;
-                } else {
-                    acc = p.wt[0] * um;
-                    acc = acc + p.wt[1] * u0;
-                    acc = acc + p.wt[2] * up;
-                    acc = acc + p.wt[3] * cm;
-                    acc = acc + p.wt[4] * c0;
-                    acc = acc + p.wt[5] * cp;
-                    acc = acc + p.wt[6] * dm;
-                    acc = acc + p.wt[7] * d0;
-                    acc = acc + p.wt[8] * dp;
-                }
-                const int gh = gy0 + y;
-                if (col_int && gh >= r && gh < H - r) Q[y * LX + x] = acc;
-                um = cm; u0 = c0; up = cp;
-                cm = dm; c0 = d0; cp = dp;
-            }
-        }
-        __syncthreads();
-    }
-
-    const float* __restrict__ R = (K & 1) ? B : A;
-    float* __restrict__ dst = (float*)s.dst;
-    for (int q = tid; q < CX * CY; q += NT) {
-        const int y = q / CX, x = q % CX;
-        const int gh = h0 + y, gw = w0 + x;
-        if (gh < r || gh >= H - r || gw < r || gw >= Wd - r) continue;
-        dst[(size_t)gh * pitch + gw] = R[(y + K) * LX + (x + 4)];
-    }
+    __shared__ float A[ttile_lds<K, CX, CY>()];
+    __shared__ float B[ttile_lds<K, CX, CY>()];
+    ttile_body<PAT, K, CX, CY>(s, p, (int)blockIdx.x, (int)blockIdx.y, K, (int)threadIdx.x, A, B);
 }
 
 template <int PAT, int K>

@@ -43,6 +43,11 @@ __global__ __launch_bounds__(1024) void k3d_plain(Sweep3D s, Points p)
     const T* __restrict__ in = (const T*)s.src;
     T* __restrict__ out = (T*)s.dst;
     const float off = s.store_offset;
+    // 16-bit scaled sums add the stored values (u - off) and fold the offset
+    // into one fused multiply-add (scaled_sum_offset_term); weighted sums
+    // decode u = x + off, as f32 storage holds u itself
+    constexpr bool DEV = ST != PSKEL_DTYPE_F32 && KIND == PSKEL_KIND_SCALED_SUM;
+    const float halo = DEV ? p.halo - off : p.halo;
     double rr = 0.0;
     if (d < D && w < W) {
         const size_t idx = ((size_t)h * W + w) * pitch + d;
@@ -59,22 +64,25 @@ __global__ __launch_bounds__(1024) void k3d_plain(Sweep3D s, Points p)
                 const int hh = h + p.dh[i], ww = w + p.dw[i], dd = d + p.dd[i];
                 float v;
                 if (!edge || ((unsigned)hh < (unsigned)H && (unsigned)ww < (unsigned)W &&
-                              (unsigned)dd < (unsigned)D))
-                    v = Tr::dec(in[((size_t)hh * W + ww) * pitch + dd], off);
-                else
-                    v = p.halo;
+                              (unsigned)dd < (unsigned)D)) {
+                    const T x = in[((size_t)hh * W + ww) * pitch + dd];
+                    v = DEV ? Tr::dec_raw(x) : Tr::dec(x, off);
+                } else {
+                    v = halo;
+                }
                 if constexpr (KIND == PSKEL_KIND_SCALED_SUM) acc = (i == 0) ? v : acc + v;
                 else acc = (i == 0) ? p.wt[i] * v : acc + p.wt[i] * v;
             }
-            if constexpr (KIND == PSKEL_KIND_SCALED_SUM) acc = p.coeff * acc;
-            if (s.rounding == PSKEL_ROUND_STOCHASTIC)
-                {
+            if constexpr (DEV) acc = __builtin_fmaf(p.coeff, acc, s.c0);
+            else if constexpr (KIND == PSKEL_KIND_SCALED_SUM) acc = p.coeff * acc;
+            if (s.rounding == PSKEL_ROUND_STOCHASTIC) {
                 const uint32_t key = sweep_key(s.seed, s.sweep);
-                res = Tr::enc_sr(acc, off, sr_cell_bits(sr_cell_key(s.seed), (uint32_t)w * D + d) ^
-                                               sr_plane_bits(key, s.plane0 + h));
+                const uint32_t rw = sr_cell_bits(sr_cell_key(s.seed), (uint32_t)w * D + d) ^
+                                    sr_plane_bits(key, s.plane0 + h);
+                res = DEV ? Tr::enc_sr_raw(acc, rw) : Tr::enc_sr(acc, off, rw);
+            } else {
+                res = DEV ? Tr::enc_raw(acc) : Tr::enc(acc, off);
             }
-            else
-                res = Tr::enc(acc, off);
         }
         out[idx] = res;
         if (s.partials) {

@@ -227,6 +227,7 @@ Sweep3D make_sweep3d(const pskel_grid_t* g, const void* src, void* dst, int radi
     s.radius = radius;
     s.store_offset = g->store_offset;
     s.partials = partials;
+    s.c0 = 0.0f;
     return s;
 }
 
@@ -320,14 +321,17 @@ struct RunPlan {
 // A 3D fused pair reduces it inside the launch (its second sweep against its
 // first); every other launch leaves only u^n and the state before the launch,
 // so in 2D a launch of k > 1 sweeps that would end on a residual step stops one
-// sweep short and the residual step runs as a single sweep.
-size_t next_k(size_t done, size_t iterations, int kmax, size_t every, int ndim)
+// sweep short and the residual step runs as a single sweep.  A 2D launch of an
+// even number of sweeps writes a spare buffer; without one (odd2d) a launch
+// fuses one sweep less.
+size_t next_k(size_t done, size_t iterations, int kmax, size_t every, int ndim, bool odd2d)
 {
     size_t k = std::min<size_t>((size_t)kmax, iterations - done);
     if (every) {
         k = std::min(k, (done / every + 1) * every - done);
         if (ndim == 2 && k > 1 && (done + k) % every == 0) k -= 1;
     }
+    if (odd2d && ndim == 2 && k % 2 == 0) k -= 1;
     return k;
 }
 
@@ -361,7 +365,7 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
     size_t done = 0;
     int64_t nl = 0, nr = 0;
     while (done < iterations) {
-        const size_t k = next_k(done, iterations, kmax, every, ndim);
+        const size_t k = next_k(done, iterations, kmax, every, ndim, g->odd_2d_launches);
         const bool want_res = every && ((done + k) % every == 0);
         const void* src = g->buf[0];
         void* dst;
@@ -415,6 +419,8 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
             }
         } else {
             Sweep3D s = make_sweep3d(g, src, dst, plan.radius, want_res ? g->d_partials : nullptr);
+            if (g->storage == PSKEL_DTYPE_BF16 || g->storage == PSKEL_DTYPE_F16)
+                s.c0 = scaled_sum_offset_term(pts.coeff, pts.np, g->store_offset);
             size_t nb;
             if (k == 2) {
                 nb = ttile3d_blocks(s, g->storage, cfg);
@@ -519,8 +525,10 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
             const int kmax = plan.pattern == PAT_LIFE8 ? 8 : 4;
             if (cfg.time_tile > kmax) return fail(PSKEL_ERR_UNSUPPORTED, "temporal depth above 4 (8 for life)");
             // a residual interval shorter than the depth only shortens the
-            // launches: none crosses a residual step (next_k)
-            plan.path = 2;
+            // launches: none crosses a residual step (next_k).  The bit-packed
+            // life tiles keep a ring of one cell: a larger mask range sweeps
+            // one generation per launch.
+            plan.path = (plan.pattern == PAT_LIFE8 && radius != 1) ? 1 : 2;
         }
         if (cfg.time_tile > 2 && ndim == 3) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 2");
         // two fused sweeps per launch run the 7-point Jacobi; for any other 3D
@@ -685,7 +693,9 @@ pskel_status_t pskel_grid_create(const pskel_grid_desc_t* desc, pskel_grid_t** o
     g->H = H;
     g->D = D;
     g->device = desc->device;
-    g->store_offset = desc->store_offset;
+    // NaN: the offset of 16-bit storage follows the data (the mean of each INPUT upload)
+    g->auto_offset = std::isnan(desc->store_offset) && (st == PSKEL_DTYPE_BF16 || st == PSKEL_DTYPE_F16);
+    g->store_offset = std::isnan(desc->store_offset) ? 0.0f : desc->store_offset;
     if (desc->rounding != PSKEL_ROUND_NEAREST && desc->rounding != PSKEL_ROUND_STOCHASTIC) {
         delete g;
         return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown rounding mode");
@@ -762,9 +772,26 @@ pskel_status_t pskel_grid_upload(pskel_grid_t* g, int32_t which, const void* hos
         for (size_t i = 0; i < n; ++i)
             if (c[i] > 1) return fail(PSKEL_ERR_INVALID_ARGUMENT, "life cells must be 0 or 1");
     }
-    if (host_dtype == g->storage) return copy_h2d(g, dbuf, host);
+    if (host_dtype == g->storage) {
+        if (g->auto_offset && which == PSKEL_BUFFER_INPUT)
+            return fail(PSKEL_ERR_INVALID_ARGUMENT, "an automatic storage offset is the mean of F32 or F64 data");
+        return copy_h2d(g, dbuf, host);
+    }
     if (host_dtype != PSKEL_DTYPE_F32 && host_dtype != PSKEL_DTYPE_F64)
         return fail(PSKEL_ERR_INVALID_ARGUMENT, "host dtype must be the storage type, F32 or F64");
+    if (g->auto_offset && which == PSKEL_BUFFER_INPUT) {
+        // the mean of the finite values, in double, rounded once
+        double sum = 0.0;
+        size_t cnt = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const double v = host_dtype == PSKEL_DTYPE_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
+            if (std::isfinite(v)) {
+                sum += v;
+                ++cnt;
+            }
+        }
+        g->store_offset = cnt ? (float)(sum / (double)cnt) : 0.0f;
+    }
     std::vector<unsigned char> tmp(n * g->elem);
     const float off = g->store_offset;
     for (size_t i = 0; i < n; ++i) {
@@ -810,6 +837,7 @@ pskel_status_t pskel_grid_info(const pskel_grid_t* g, pskel_grid_info_t* info)
     info->depth = g->D;
     info->pitch = g->pitch;
     info->bytes_per_buffer = g->bytes;
+    info->store_offset = g->store_offset;
     info->current = g->buf[0];
     info->previous = g->buf[1];
     info->stream = g->stream;
@@ -934,7 +962,7 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
         // advance the host-side state exactly as the captured sequence did
         const int kmax = (plan.path == 2) ? std::max(1, cfg.time_tile) : 1;
         for (size_t done = 0; done < iterations;) {
-            const size_t k = next_k(done, iterations, kmax, every, g->ndim);
+            const size_t k = next_k(done, iterations, kmax, every, g->ndim, g->odd_2d_launches);
             if (k % 2 == 0 && g->ndim == 2) g->spare_parity = g->parity;  // ring copy in the graph
             advance_state(*g, k);
             g->sweeps_done += k;
@@ -955,6 +983,10 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
     }
     return PSKEL_OK;
 }
+
+}  // extern "C"
+
+extern "C" {
 
 pskel_status_t pskel_grid_swap(pskel_grid_t* g)
 {

@@ -75,6 +75,7 @@ struct Grid {
     size_t nrows = 0;   // 2D: H, 3D: H*W
     size_t bytes = 0;   // per buffer
     float store_offset = 0.0f;
+    bool auto_offset = false;  // store_offset is the mean of the last INPUT upload
     int device = 0;
     int rounding = PSKEL_ROUND_NEAREST;
     uint32_t seed = 0;
@@ -95,6 +96,7 @@ struct Grid {
     size_t res_cap = 0;
 
     pskel_field_t* fields[4] = {nullptr, nullptr, nullptr, nullptr};  // attached field arguments
+    bool odd_2d_launches = false;  // 2D launches fuse an odd number of sweeps (no spare buffer)
 };
 
 void advance_state(Grid& g, size_t k);
@@ -123,7 +125,18 @@ struct Sweep3D {
     uint32_t seed;
     uint64_t sweep;     // index of the sweep this launch performs
     uint32_t plane0;    // global h of plane 0 (stochastic rounding words use the global plane)
+    float c0;           // 16-bit scaled sums: scaled_sum_offset_term of the stencil and offset
 };
+
+// 16-bit storage holds x = u - off.  A scaled sum of n stored cells is then
+// coeff * sum(x_i + off) - off = coeff * sum(x_i) + (coeff * n - 1) * off, so
+// the kernels sum the stored values as they are and add this term once, in
+// one fused multiply-add: fma(coeff, sum(x_i), c0).  It is evaluated in double
+// and rounded once to float, on the host.
+inline float scaled_sum_offset_term(float coeff, int npoints, float off)
+{
+    return (float)(((double)coeff * (double)npoints - 1.0) * (double)off);
+}
 
 // CLOUDSIM field arguments as its kernels see them (reversed device layout:
 // the field value of cell (h, w) at h*fpitch + w).

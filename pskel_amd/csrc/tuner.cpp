@@ -7,10 +7,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
 #include <mutex>
 #include <string>
-#include <cmath>
-#include <cstring>
 #include <vector>
 
 #include "pskel_internal.hpp"
@@ -32,14 +35,22 @@ std::once_flag g_model_once;
 std::mutex g_predict_mutex;
 
 // The shipped cost model lives next to the library: <libdir>/data/costmodel.bin.
+// PSKEL_TUNER_MODEL names another model file, or "none" for no model (the
+// tuner then searches by measurement).
 void load_shipped_model()
 {
-    Dl_info info;
-    if (!dladdr((void*)&load_shipped_model, &info) || !info.dli_fname) return;
-    std::string dir(info.dli_fname);
-    const size_t slash = dir.rfind('/');
-    dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash);
-    const std::string path = dir + "/data/costmodel.bin";
+    std::string path;
+    if (const char* env = std::getenv("PSKEL_TUNER_MODEL")) {
+        if (std::strcmp(env, "none") == 0 || env[0] == '\0') return;
+        path = env;
+    } else {
+        Dl_info info;
+        if (!dladdr((void*)&load_shipped_model, &info) || !info.dli_fname) return;
+        std::string dir(info.dli_fname);
+        const size_t slash = dir.rfind('/');
+        dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash);
+        path = dir + "/data/costmodel.bin";
+    }
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) return;
     std::fclose(f);
@@ -90,11 +101,11 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
         const int ry_nwy[][2] = {{1, 2}, {1, 4}, {2, 2}, {2, 4}, {4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}};
         for (auto& t : ry_nwy) v.push_back(tuned(0, t[0], t[1], 1));
         if (pat == PAT_LIFE8) {  // bit-packed tiles: rows per lane x generations per launch
-            for (int k : {2, 4, 6, 8})
+            for (int k = 2; k <= 8; ++k)
                 for (int rows : {32, 40, 48}) v.push_back(tuned(0, rows, 0, k));
         } else if (pat != PAT_CLOUD) {  // field stencils run one sweep per launch
             for (int k = 2; k <= 4; ++k)
-                for (int cy : {16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
+                for (int cy : {8, 16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
         }
         v.push_back(plain(64, 4));
     } else if (q.ndim == 3 && pat == PAT_J7) {
@@ -239,6 +250,128 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
     return plain(64, 4);
 }
 
+// ---------------------------------------------------------------------------
+// Measured search: the fallback without a cost model and the yardstick of its
+// regret.  Times are device time per sweep (pskel_tuner_measure); a candidate
+// the grid cannot run (a launch error) costs +inf.
+// ---------------------------------------------------------------------------
+struct Timer {
+    pskel_grid_t* g;
+    const pskel_stencil_desc_t* d;
+    size_t sweeps;
+    int reps;
+    const std::vector<pskel_launch_cfg_t>* cands;
+    std::vector<double> t;  // NaN: not timed yet
+    int timed = 0;
+    double operator()(size_t i)
+    {
+        if (std::isnan(t[i])) {
+            double ms = 0.0;
+            const pskel_status_t st = pskel_tuner_measure(g, d, sweeps, &(*cands)[i], reps, &ms);
+            t[i] = st == PSKEL_OK ? ms : std::numeric_limits<double>::infinity();
+            ++timed;
+        }
+        return t[i];
+    }
+};
+
+size_t search_exhaustive(Timer& time)
+{
+    size_t best = 0;
+    for (size_t i = 0; i < time.t.size(); ++i)
+        if (time(i) < time(best)) best = i;
+    return best;
+}
+
+// A simple GA over candidate indices, the set-up of the reference's GAlib
+// search (include/PSkelStencil.hpp:1910-1940: simple GA, binary genome,
+// sigma-truncation scaling, one-point crossover, bit-flip mutation) but scoring
+// measured time and with a fixed seed (the reference seeds with time(NULL)).
+size_t search_genetic(Timer& time, uint64_t seed)
+{
+    const size_t n = time.t.size();
+    int nbits = 1;
+    while ((size_t(1) << nbits) < n) ++nbits;
+    constexpr int kPop = 16, kGen = 8;
+    constexpr double kCross = 0.6, kMut = 0.01;
+    uint64_t rs = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto rnd = [&rs]() {  // splitmix64 -> [0, 1)
+        uint64_t z = (rs += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto decode = [&](uint32_t g) { return (size_t)g % n; };
+    std::vector<uint32_t> pop(kPop);
+    for (auto& g : pop) {
+        g = 0;
+        for (int b = 0; b < nbits; ++b) g |= (rnd() < 0.5 ? 1u : 0u) << b;
+    }
+    size_t best = decode(pop[0]);
+    for (auto g : pop)
+        if (time(decode(g)) < time(best)) best = decode(g);
+    for (int gen = 0; gen < kGen; ++gen) {
+        // sigma truncation of the score -time: f = max(score - (mean - 2 sd), 0)
+        std::vector<double> score(kPop);
+        double mean = 0.0;
+        for (int i = 0; i < kPop; ++i) {
+            const double t = time(decode(pop[i]));
+            score[i] = std::isfinite(t) ? -t : -1e30;
+            mean += score[i];
+        }
+        mean /= kPop;
+        double var = 0.0;
+        for (double x : score) var += (x - mean) * (x - mean);
+        const double sd = std::sqrt(var / kPop);
+        std::vector<double> cum(kPop);
+        double acc = 0.0;
+        for (int i = 0; i < kPop; ++i) {
+            acc += std::max(score[i] - (mean - 2.0 * sd), 0.0) + 1e-12;
+            cum[i] = acc;
+        }
+        auto pick = [&]() {
+            const double x = rnd() * acc;
+            int i = 0;
+            while (i < kPop - 1 && cum[i] < x) ++i;
+            return pop[i];
+        };
+        std::vector<uint32_t> next;
+        while ((int)next.size() < kPop) {
+            uint32_t a = pick(), b = pick();
+            if (rnd() < kCross && nbits > 1) {
+                const int cut = 1 + (int)(rnd() * (nbits - 1));
+                const uint32_t lo = (1u << cut) - 1u;
+                const uint32_t na = (a & lo) | (b & ~lo), nb = (b & lo) | (a & ~lo);
+                a = na;
+                b = nb;
+            }
+            for (int bit = 0; bit < nbits; ++bit) {
+                if (rnd() < kMut) a ^= 1u << bit;
+                if (rnd() < kMut) b ^= 1u << bit;
+            }
+            next.push_back(a & ((1u << nbits) - 1u));
+            next.push_back(b & ((1u << nbits) - 1u));
+        }
+        pop.assign(next.begin(), next.begin() + kPop);
+        for (auto g : pop)
+            if (time(decode(g)) < time(best)) best = decode(g);
+    }
+    return best;
+}
+
+// Search results of this process, so a shape is searched once.
+struct SearchKey {
+    std::vector<unsigned char> bytes;
+    bool operator<(const SearchKey& o) const { return bytes < o.bytes; }
+};
+struct SearchResult {
+    pskel_launch_cfg_t cfg;
+    double ms;
+    int32_t timed;
+};
+std::mutex g_search_mutex;
+std::map<SearchKey, SearchResult> g_search_cache;
+
 pskel_status_t check_query(const pskel_tune_query_t* q, const pskel_stencil_desc_t* d)
 {
     if (!q || !d) return fail(PSKEL_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -318,6 +451,66 @@ pskel_status_t pskel_tuner_choose(const pskel_mlp_t* model, const pskel_tune_que
     if (st != PSKEL_OK) return st;
     const size_t best = (size_t)(std::min_element(y.begin(), y.end()) - y.begin());
     *out = v[best];
+    return PSKEL_OK;
+}
+
+pskel_status_t pskel_tuner_has_model(int32_t* out)
+{
+    if (!out) return fail(PSKEL_ERR_INVALID_ARGUMENT, "out is NULL");
+    std::call_once(g_model_once, load_shipped_model);
+    std::lock_guard<std::mutex> lock(g_predict_mutex);
+    *out = g_default_model ? 1 : 0;
+    return PSKEL_OK;
+}
+
+pskel_status_t pskel_tuner_search(pskel_grid_t* g, const pskel_stencil_desc_t* d, size_t iterations, int32_t method,
+                                  size_t sweeps, int32_t reps, pskel_launch_cfg_t* out, double* ms_per_sweep,
+                                  int32_t* measured)
+{
+    if (!g || !d || !out) return fail(PSKEL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (method != PSKEL_SEARCH_EXHAUSTIVE && method != PSKEL_SEARCH_GENETIC)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown search method");
+    pskel_grid_info_t gi{};
+    pskel_status_t st = pskel_grid_info(g, &gi);
+    if (st != PSKEL_OK) return st;
+    pskel_tune_query_t q{};
+    q.ndim = gi.ndim;
+    q.storage = gi.storage;
+    q.width = gi.width;
+    q.height = gi.height;
+    q.depth = gi.depth;
+    q.iterations = std::max<size_t>(iterations, 1);
+    if ((st = check_query(&q, d)) != PSKEL_OK) return st;
+    if (sweeps == 0) sweeps = std::min<size_t>(q.iterations, 24);
+    reps = std::max(reps, 1);
+    SearchKey key;
+    auto put = [&key](const void* p, size_t n) {
+        key.bytes.insert(key.bytes.end(), (const unsigned char*)p, (const unsigned char*)p + n);
+    };
+    put(&q, sizeof(q));
+    put(d, sizeof(*d));
+    put(&method, sizeof(method));
+    put(&sweeps, sizeof(sweeps));
+    {
+        std::lock_guard<std::mutex> lock(g_search_mutex);
+        auto it = g_search_cache.find(key);
+        if (it != g_search_cache.end()) {
+            *out = it->second.cfg;
+            if (ms_per_sweep) *ms_per_sweep = it->second.ms;
+            if (measured) *measured = 0;
+            return PSKEL_OK;
+        }
+    }
+    const auto v = candidates(q, *d);
+    Timer time{g, d, sweeps, reps, &v, std::vector<double>(v.size(), std::nan("")), 0};
+    const size_t best = method == PSKEL_SEARCH_EXHAUSTIVE ? search_exhaustive(time) : search_genetic(time, 1);
+    if (!std::isfinite(time(best))) return fail(PSKEL_ERR_RUNTIME, "no candidate ran on this grid");
+    set_error("");
+    *out = v[best];
+    if (ms_per_sweep) *ms_per_sweep = time(best);
+    if (measured) *measured = time.timed;
+    std::lock_guard<std::mutex> lock(g_search_mutex);
+    g_search_cache[key] = SearchResult{v[best], time(best), time.timed};
     return PSKEL_OK;
 }
 

@@ -227,16 +227,22 @@ class CostModel:
         os.makedirs(os.path.dirname(path), exist_ok=True)
         L.check(L.lib().pskel_mlp_save(self._h, path.encode()))
 
+    @staticmethod
+    def normalization(X: np.ndarray, y: np.ndarray):
+        """Per-feature mean and spread of a training table, and of its targets."""
+        mean = X.mean(0).astype(np.float32)
+        sd = X.std(0).astype(np.float32)
+        sd[sd < 1e-6] = 1.0
+        return mean, sd, float(y.mean()), float(max(y.std(), 1e-6))
+
     def fit(self, X: np.ndarray, y: np.ndarray, epochs: int = 400, batch: int = 128, lr: float = 2e-3,
             seed: int = 1) -> float:
         X = np.ascontiguousarray(X, np.float32)
         y = np.ascontiguousarray(y, np.float32)
-        mean = X.mean(0).astype(np.float32)
-        sd = X.std(0).astype(np.float32)
-        sd[sd < 1e-6] = 1.0
+        mean, sd, ym, ys = self.normalization(X, y)
         F = C.POINTER(C.c_float)
         L.check(L.lib().pskel_mlp_set_normalization(self._h, mean.ctypes.data_as(F), sd.ctypes.data_as(F),
-                                                    float(y.mean()), float(max(y.std(), 1e-6))))
+                                                    ym, ys))
         loss = C.c_float()
         L.check(L.lib().pskel_mlp_train(self._h, X.ctypes.data_as(F), y.ctypes.data_as(F), len(y), epochs, batch,
                                         lr, seed, C.byref(loss)))
@@ -259,11 +265,67 @@ class CostModel:
         """Make this the model pskel_tuner_choose(NULL, ...) uses."""
         L.check(L.lib().pskel_tuner_set_default_model(self._h))
 
+    def params(self) -> np.ndarray:
+        n = C.c_size_t()
+        L.check(L.lib().pskel_mlp_num_params(self._h, C.byref(n)))
+        p = np.zeros(n.value, np.float32)
+        L.check(L.lib().pskel_mlp_get_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float)), n.value))
+        return p
+
     def close(self):
         if getattr(self, "_h", None):
             L.lib().pskel_tuner_set_default_model(None)
             L.lib().pskel_mlp_destroy(self._h)
             self._h = None
+
+
+def pack_ensemble(members: Sequence[CostModel], mean: np.ndarray, sd: np.ndarray, y_mean: float,
+                  y_sd: float) -> CostModel:
+    """One MLP whose output is the mean of the members' outputs: the members'
+    hidden layers side by side (block-diagonal weights), the output layer's
+    weights and bias divided by the member count.  Members share the input
+    and output normalisation (trained on the same table) and the layer sizes
+    (in, h1, ..., 1); the result has layer sizes (in, M*h1, ..., 1), so it is an
+    ordinary model file for pskel_tuner_choose."""
+    big, flat = pack_params([m.params() for m in members], members[0].sizes)
+    model = CostModel(sizes=big)
+    F = C.POINTER(C.c_float)
+    mean = np.ascontiguousarray(mean, np.float32)
+    sd = np.ascontiguousarray(sd, np.float32)
+    L.check(L.lib().pskel_mlp_set_normalization(model._h, mean.ctypes.data_as(F), sd.ctypes.data_as(F),
+                                                float(y_mean), float(y_sd)))
+    L.check(L.lib().pskel_mlp_set_params(model._h, flat.ctypes.data_as(F), flat.size))
+    return model
+
+
+def pack_params(ps: Sequence[np.ndarray], sizes: Sequence[int]) -> Tuple[List[int], np.ndarray]:
+    """Flat parameters of the averaged ensemble (see pack_ensemble) from the
+    members' flat parameters, each laid out layer by layer as W (out x in,
+    row-major) then b."""
+    M = len(ps)
+    nl = len(sizes) - 1
+    big = [sizes[0]] + [M * h for h in sizes[1:-1]] + [1]
+    out = []
+    offs = [0]
+    for l in range(nl):
+        offs.append(offs[-1] + sizes[l + 1] * sizes[l] + sizes[l + 1])
+    for l in range(nl):
+        W = np.zeros((big[l + 1], big[l]), np.float32)
+        b = np.zeros(big[l + 1], np.float32)
+        for j, p in enumerate(ps):
+            w = p[offs[l]:offs[l] + sizes[l + 1] * sizes[l]].reshape(sizes[l + 1], sizes[l])
+            bb = p[offs[l] + sizes[l + 1] * sizes[l]:offs[l + 1]]
+            if l == nl - 1:
+                W[:, j * sizes[l]:(j + 1) * sizes[l]] = w / M
+                b += bb / M
+            elif l == 0:
+                W[j * sizes[1]:(j + 1) * sizes[1], :] = w
+                b[j * sizes[1]:(j + 1) * sizes[1]] = bb
+            else:
+                W[j * sizes[l + 1]:(j + 1) * sizes[l + 1], j * sizes[l]:(j + 1) * sizes[l]] = w
+                b[j * sizes[l + 1]:(j + 1) * sizes[l + 1]] = bb
+        out += [W.ravel(), b]
+    return big, np.ascontiguousarray(np.concatenate(out), np.float32)
 
 
 def heuristic_choice(q: Query) -> L.LaunchCfg:
@@ -326,6 +388,16 @@ def collect(queries: Sequence[Query], reps: int = 3, log=None) -> List[Dict]:
             log(f"collected {qi + 1}/{len(queries)} {q.stencil_name} {q.shape} {q.storage}: "
                 f"{len(cands)} candidates, best {min(r['ms'] for r in rows[-len(cands):]):.4f} ms/sweep")
     return rows
+
+
+def rows_to_json(rows: Sequence[Dict]) -> List[Dict]:
+    return [{"shape": list(r["q"].shape), "storage": r["q"].storage, "stencil": r["q"].stencil_name,
+             "iterations": r["q"].iterations, "ci": r["ci"], "ms": r["ms"]} for r in rows]
+
+
+def rows_from_json(items: Sequence[Dict]) -> List[Dict]:
+    return [{"q": Query(tuple(x["shape"]), x["storage"], x["stencil"], x["iterations"]), "ci": x["ci"],
+             "ms": x["ms"]} for x in items]
 
 
 def dataset(rows: Sequence[Dict]) -> Tuple[np.ndarray, np.ndarray]:

@@ -1,13 +1,17 @@
 """Train the launch cost model on timings measured on this GPU.
 
     python -m pskel_amd.tuner.train [--train N] [--test M] [--out pskel_amd/data/costmodel.bin]
-                                    [--report path.json]
+                                    [--report path.json] [--ensemble E]
+                                    [--save-rows rows.json | --load-rows rows.json]
 
 Collects exhaustive timings of every candidate for N random queries (about 20-40
-candidates each), trains the MLP with the library's HIP kernels, then measures M
-held-out queries exhaustively and reports, per query, the model's regret
-(time of its choice / best time - 1), the GA's regret and measurement count,
-and the speed-up of the model's choice over the plain kernel.
+candidates each), trains E MLPs from different initialisations with the
+library's HIP kernels and ships their mean (packed into one model file), then
+measures M held-out queries exhaustively and reports, per query, the model's
+regret (time of its choice / best time - 1), the heuristic's and the GA's
+regret and measurement count, and the speed-up of the model's choice over the
+plain kernel.  --save-rows keeps both timing tables; --load-rows trains and
+reports from a saved pair instead of measuring again.
 """
 from __future__ import annotations
 
@@ -20,7 +24,7 @@ import time
 import numpy as np
 
 from . import (CostModel, Measurer, candidates, cfg_dict, cfg_key, collect, dataset, exhaustive, genetic,
-               random_queries, regret, DEFAULT_MODEL)
+               pack_ensemble, random_queries, regret, rows_from_json, rows_to_json, DEFAULT_MODEL)
 
 
 def main(argv=None):
@@ -31,23 +35,47 @@ def main(argv=None):
     ap.add_argument("--out", default=DEFAULT_MODEL)
     ap.add_argument("--report", default="")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--ensemble", type=int, default=5, help="members averaged into the shipped model")
+    ap.add_argument("--save-rows", default="")
+    ap.add_argument("--load-rows", default="")
     args = ap.parse_args(argv)
 
     log = lambda m: print(m, flush=True)  # noqa: E731
     t0 = time.time()
-    train_q = random_queries(args.train, args.seed)
-    rows = collect(train_q, log=log if args.train <= 400 else None)
+    if args.load_rows:
+        with open(args.load_rows) as f:
+            saved = json.load(f)
+        rows, test_rows = rows_from_json(saved["train"]), rows_from_json(saved["test"])
+        train_q = list(dict.fromkeys(r["q"] for r in rows))
+        test_q = list(dict.fromkeys(r["q"] for r in test_rows))
+    else:
+        train_q = random_queries(args.train, args.seed)
+        rows = collect(train_q, log=log if args.train <= 400 else None)
+        test_q = random_queries(args.test, args.seed + 1)
+        test_rows = collect(test_q)
+        if args.save_rows:
+            with open(args.save_rows, "w") as f:
+                json.dump({"train": rows_to_json(rows), "test": rows_to_json(test_rows)}, f)
     X, y = dataset(rows)
     log(f"training set: {len(y)} timings from {len(train_q)} queries ({time.time() - t0:.0f} s)")
-    model = CostModel(seed=args.seed)
-    loss = model.fit(X, y, epochs=args.epochs)
+    members = []
+    Xt, yt = dataset(test_rows)
+    member_regret = []
+    for j in range(max(1, args.ensemble)):
+        m = CostModel(seed=args.seed + 7919 * j)
+        loss = m.fit(X, y, epochs=args.epochs, seed=args.seed + 104729 * j)
+        pred = m.predict(X)
+        log(f"member {j}: train loss {loss:.4f} (standardised), rms log-error "
+            f"{np.sqrt(np.mean((pred - y) ** 2)):.3f}")
+        member_regret.append(float(np.mean([x["regret"] for x in regret(m, test_rows)])))
+        members.append(m)
+    model = pack_ensemble(members, *CostModel.normalization(np.asarray(X, np.float32), np.asarray(y, np.float32)))
+    for m in members:
+        m.close()
     pred = model.predict(X)
-    log(f"train loss {loss:.4f} (standardised), rms log-error {np.sqrt(np.mean((pred - y) ** 2)):.3f}")
+    log(f"ensemble of {len(members)}: rms log-error {np.sqrt(np.mean((pred - y) ** 2)):.3f}")
     model.save(args.out)
 
-    test_q = random_queries(args.test, args.seed + 1)
-    test_rows = collect(test_q)
-    Xt, yt = dataset(test_rows)
     pt = model.predict(Xt)
     reg = regret(model, test_rows)
     # GA on the held-out queries (measured objective, cached timings)
@@ -70,6 +98,8 @@ def main(argv=None):
         "test_queries": len(test_q),
         "test_timings": int(len(yt)),
         "test_rms_log_error": float(np.sqrt(np.mean((pt - yt) ** 2))),
+        "ensemble_members": len(members),
+        "member_regret_means": member_regret,
         "model_regret_mean": float(r.mean()),
         "model_regret_median": float(np.median(r)),
         "model_regret_p90": float(np.percentile(r, 90)),

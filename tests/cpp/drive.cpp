@@ -14,7 +14,10 @@
 // 1.0) and `-` for the result file (not written).
 // Environment: PSKEL_DRIVE_RESIDUAL=n sets setResidualInterval(n) and prints
 // "residuals r1 r2 ..." on stderr; PSKEL_DRIVE_TIMES prints "gpu_ms <device
-// time of the sweeps> wall_ms <whole run method, host copies included>".
+// time of the sweeps> wall_ms <whole run method, host copies included>";
+// PSKEL_DRIVE_REPEAT=n runs the method n times (the GPU methods give the same
+// result each time: they leave `input` alone), timing each.
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -124,7 +127,22 @@ static bool writef(const char* p, const T* src, size_t n)
 }
 
 template <class St>
+static int runMethodOnce(St& st, const std::string& method, size_t iters);
+
+template <class St>
 static int runMethod(St& st, const std::string& method, size_t iters)
+{
+    const char* rep = std::getenv("PSKEL_DRIVE_REPEAT");
+    const long n = rep ? std::max(1L, std::strtol(rep, nullptr, 10)) : 1L;
+    for (long i = 0; i < n; ++i) {
+        const int rc = runMethodOnce(st, method, iters);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+template <class St>
+static int runMethodOnce(St& st, const std::string& method, size_t iters)
 {
 #ifdef PSKEL_GPU
     if (const char* r = std::getenv("PSKEL_DRIVE_RESIDUAL")) st.setResidualInterval(std::strtoull(r, nullptr, 10));

@@ -49,6 +49,44 @@ def test_batch_matches_oracle(gpu, use_graph, rows):
             g.close()
 
 
+DEPTH_SHAPES = [(64, 64, "jacobi"), (70, 1030, "life"), (257, 66, "jacobi"), (1000, 300, "life"), (3, 9, "jacobi"),
+                (129, 513, "diff9"), (2048, 64, "life"), (333, 777, "jacobi"), (40, 2100, "life"), (500, 130, "diff9"),
+                (2, 2, "life"), (5, 3000, "diff9")]
+DEPTHS = [3, 7, 1, 5, 3, 3, 3, 3, 7, 1, 5, 3]
+ROWS = [16, 0, 4, 0, 32, 16, 0, 32, 0, 2, 0, 16]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_temporal_depths(gpu, use_graph):
+    """Per-grid depths (3 for f32 LDS tiles, 3/5/7 for bit-packed life, 1 for
+    register tiles) and runs of 13, 8 and 1 sweeps, whose remainders take no,
+    one or two tail launches: every grid equals the oracle after each run."""
+    grids, stencils, fields = _make(DEPTH_SHAPES, seed=7)
+    try:
+        with api.Batch(grids, stencils, ROWS, DEPTHS) as b:
+            done = 0
+            for n in (13, 8, 1, 14):
+                st = b.run(n, use_graph=use_graph)
+                done += n
+                assert st["launches"] == max(-(-n // k) + (1 if n % k and (n % k) % 2 == 0 else 0) for k in DEPTHS)
+                for g, s, f, k in zip(grids, stencils, fields, DEPTHS):
+                    assert np.array_equal(g.download(), oracle.run_2d(s, f, done)), (g.shape, k, done)
+    finally:
+        for g in grids:
+            g.close()
+
+
+def test_batch_depth_validation(gpu):
+    grids, stencils, _ = _make([(64, 64, "jacobi"), (64, 64, "life")], seed=9)
+    try:
+        for depths, rows in (([2, 1], None), ([5, 1], None), ([1, 4], None), ([3, 1], [8, 4]), ([1, 9], None)):
+            with pytest.raises(PSkelError):
+                api.Batch(grids, stencils, rows, depths)
+    finally:
+        for g in grids:
+            g.close()
+
+
 def test_batch_graph_replays(gpu):
     """The same captured graph replayed several times, with eager work between."""
     grids, stencils, fields = _make(SHAPES[:5], seed=3)

@@ -92,3 +92,42 @@ def test_host_tiled_rejects(gpu):
     t, wx, wy = inputs.cloudsim_fields(20, 30, seed=9)
     with pytest.raises(PSkelError):
         api.run_host_tiled(K.cloudsim(), t, 3, 4, 1)
+
+
+def _host_search(a, stencil, iterations, budget=0):
+    import ctypes as C
+
+    from pskel_amd import _lib as L
+
+    gd = L.GridDesc()
+    gd.ndim = a.ndim
+    gd.storage = L.DTYPE_F32 if a.dtype == np.float32 else L.DTYPE_U8
+    gd.height, gd.width = a.shape[0], a.shape[1]
+    gd.depth = a.shape[2] if a.ndim == 3 else 1
+    d = stencil.desc()
+    tile, inner, ms = C.c_size_t(), C.c_size_t(), C.c_double()
+    L.check(L.lib().pskel_tuner_host_search(C.byref(gd), np.ascontiguousarray(a).ctypes.data, gd.storage,
+                                            C.byref(d), iterations, budget, C.byref(tile), C.byref(inner),
+                                            C.byref(ms)))
+    return tile.value, inner.value
+
+
+def test_out_of_core_search_picks_a_runnable_launch(gpu):
+    """pskel_tuner_host_search (runIterativeAutoGPU's tiled branch): the slab
+    height and sweeps per pass it returns are candidates, fit the budget, and
+    give the oracle's result."""
+    a = inputs.jacobi3d_field(96, seed=33)
+    tile, inner = _host_search(a, K.jacobi3d(), 20)
+    assert tile in (16, 32, 64) and inner in (2, 4, 8, 16)
+    got, _ = api.run_host_tiled(K.jacobi3d(), a, 20, tile, inner)
+    want, _ = oracle.run_3d(K.jacobi3d(), a, 20)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # a device budget too small for any slab is refused
+    from pskel_amd._lib import PSkelError
+
+    with pytest.raises(PSkelError):
+        _host_search(a, K.jacobi3d(), 20, budget=1 << 16)
+    b = inputs.life_field(600, seed=34)
+    tile, inner = _host_search(b, K.life(), 12)
+    got, _ = api.run_host_tiled(K.life(), b, 12, tile, inner)
+    assert np.array_equal(got, oracle.run_2d(K.life(), b, 12))

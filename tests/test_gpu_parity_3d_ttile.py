@@ -247,3 +247,25 @@ def test_fused_host_staged(gpu, store):
         got, _ = api.run_host_tiled(K.jacobi3d(), words, 10, 9, 4, storage="bf16", store_offset=0.5,
                                     rounding="stochastic", seed=77, time_tile=2)
         assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("store", ["bf16", "f16"])
+def test_automatic_storage_offset(gpu, store):
+    """store_offset "auto" (NaN in the C ABI): an INPUT upload sets the offset
+    to the mean of the uploaded values (summed in double in order, rounded
+    once), and the run equals one with that offset given explicitly; raw
+    storage words cannot set it."""
+    a = _field((20, 30, 70), 31)
+    mean = np.float32(np.cumsum(a.ravel().astype(np.float64))[-1] / a.size)
+    with api.Grid(a.shape, store, store_offset="auto", rounding="stochastic", seed=3) as g:
+        g.upload_pair(a)
+        assert np.float32(g.info()["store_offset"]) == mean
+        g.run(K.jacobi3d(), 7, variant="tuned", time_tile=2)
+        got = g.download(raw=True)
+        with pytest.raises(PSkelError):
+            g.upload(0, np.zeros(a.shape, np.uint16))
+    with api.Grid(a.shape, store, store_offset=float(mean), rounding="stochastic", seed=3) as g:
+        g.upload_pair(a)
+        g.run(K.jacobi3d(), 7, variant="tuned", time_tile=2)
+        want = g.download(raw=True)
+    assert np.array_equal(got, want)

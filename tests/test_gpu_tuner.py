@@ -1,5 +1,6 @@
 """The learned launch tuner on the GPU: collection, training by the device
-kernels, choice, regret, and the gain over the plain kernel."""
+kernels, choice, regret against exhaustive search, the gain over the
+library's fixed heuristic, and the C ABI's measured search fallback."""
 import os
 
 import numpy as np
@@ -50,23 +51,99 @@ def test_genetic_search_finds_a_good_candidate(gpu):
 
 
 @pytest.mark.skipif(not os.path.exists(T.DEFAULT_MODEL), reason="no shipped cost model")
-def test_shipped_model_regret_and_gain(gpu):
-    """The shipped model on fresh queries (not its training set): its choice is
-    within 25% of the exhaustive best on every query and clearly faster than
-    the plain kernel overall."""
+def test_shipped_model_regret_on_the_workloads(gpu):
+    """The shipped model on the benchmark workloads' shapes (not its training
+    set): its choice is within 10% of the exhaustive best on every query."""
     m = T.CostModel.load()
     try:
         qs = [T.Query((2048, 2048), "f32", "jacobi2d", 100), T.Query((8194, 8194), "u8", "life", 500),
               T.Query((4096, 4096), "f32", "diffusion9", 1000), T.Query((512, 512, 512), "f32", "jacobi3d", 1000),
-              T.Query((700, 333), "f32", "jacobi2d", 40), T.Query((96, 200, 400), "bf16", "jacobi3d", 100)]
-        rows = T.collect(qs, reps=2)
-        reg = T.regret(m, rows)
-        for x in reg:
-            assert x["regret"] <= 0.25, (x["q"], x["regret"])
-        gain = np.exp(np.mean([np.log(x["plain_ms"] / x["model_ms"]) for x in reg]))
-        assert gain > 1.5, gain
+              T.Query((512, 512, 512), "bf16", "jacobi3d", 1000), T.Query((700, 333), "f32", "jacobi2d", 40),
+              T.Query((96, 200, 400), "bf16", "jacobi3d", 100)]
+        rows = T.collect(qs, reps=3)
+        for x in T.regret(m, rows):
+            assert x["regret"] <= 0.10, (x["q"], x["regret"])
     finally:
         m.close()
+
+
+@pytest.mark.skipif(not os.path.exists(T.DEFAULT_MODEL), reason="no shipped cost model")
+def test_shipped_model_gain_over_heuristic_on_fresh_queries(gpu):
+    """Random queries from a seed neither the training nor the held-out set
+    used: the model's choices, over the exhaustive table, against the fixed
+    heuristic the library falls back to.  The model's mean regret stays under
+    10% and its choices are at least as fast as the heuristic's overall
+    (geometric mean, within 2% for timing noise)."""
+    m = T.CostModel.load()
+    try:
+        qs = T.random_queries(12, 4242)
+        rows = T.collect(qs, reps=3)
+        reg = T.regret(m, rows)
+        mean_model = float(np.mean([x["regret"] for x in reg]))
+        gain = float(np.exp(np.mean([np.log(x["heuristic_ms"] / x["model_ms"]) for x in reg])))
+        print(f"model mean regret {mean_model:.4f}, heuristic mean regret "
+              f"{np.mean([x['heuristic_regret'] for x in reg]):.4f}, model gain over heuristic {gain:.4f}")
+        assert mean_model <= 0.10, mean_model
+        assert gain >= 0.98, gain
+    finally:
+        m.close()
+
+
+def _grid_for(q):
+    from pskel_amd import api
+
+    g = api.Grid(q.shape, q.storage)
+    g.upload_pair(T.field_for(q))
+    return g
+
+
+def test_library_search_exhaustive_and_genetic(gpu):
+    """pskel_tuner_search (the C ABI's own fallback): exhaustive returns the
+    fastest candidate it timed, the GA times at most every candidate and lands
+    near it, and a repeated search is answered from the process cache."""
+    import ctypes as C
+
+    q = T.Query((600, 900), "f32", "diffusion9", 48)
+    d = q.stencil().desc()
+    n_cands = len(T.candidates(q))
+    keys = {T.cfg_key(c) for c in T.candidates(q)}
+    out, ms, nm = L.LaunchCfg(), C.c_double(), C.c_int32()
+    g = _grid_for(q)
+    try:
+        L.check(L.lib().pskel_tuner_search(g._h, C.byref(d), 48, L.SEARCH_EXHAUSTIVE, 24, 3, C.byref(out),
+                                           C.byref(ms), C.byref(nm)))
+        best, t_ex = T.cfg_key(out), ms.value
+        assert best in keys and nm.value == n_cands and t_ex > 0
+        L.check(L.lib().pskel_tuner_search(g._h, C.byref(d), 48, L.SEARCH_EXHAUSTIVE, 24, 3, C.byref(out),
+                                           C.byref(ms), C.byref(nm)))
+        assert nm.value == 0 and T.cfg_key(out) == best  # cached
+        L.check(L.lib().pskel_tuner_search(g._h, C.byref(d), 48, L.SEARCH_GENETIC, 24, 3, C.byref(out),
+                                           C.byref(ms), C.byref(nm)))
+        assert T.cfg_key(out) in keys and 1 <= nm.value <= n_cands
+        assert ms.value <= 1.5 * t_ex
+    finally:
+        g.close()
+
+
+def test_dropin_auto_without_model_searches(gpu, monkeypatch):
+    """PSKEL_TUNER_MODEL=none: the drop-in's runIterativeAutoGPU and
+    runIterativeGPU(N) time the candidates on the grid, start again from the
+    host state and give the oracle's result."""
+    import dropin
+    import oracle
+    from pskel_amd import inputs
+    from pskel_amd import kernels as K
+
+    monkeypatch.setenv("PSKEL_TUNER_MODEL", "none")
+    a3 = inputs.jacobi3d_field(40, seed=61)
+    a2 = inputs.jacobi2d_field(150, 260, seed=62)
+    for method in ("auto", "default"):
+        rc, got, err = dropin.run("drive_gpu_gxx", "jacobi3d", method, a3, 9)
+        assert rc == 0, err
+        assert np.array_equal(got.view(np.uint32), oracle.run_3d(K.jacobi3d(), a3, 9)[0].view(np.uint32))
+        rc, got, err = dropin.run("drive_gpu_gxx", "jacobi2d", method, a2, 9)
+        assert rc == 0, err
+        assert np.array_equal(got.view(np.uint32), oracle.run_2d(K.jacobi2d(), a2, 9).view(np.uint32))
 
 
 def test_auto_gpu_uses_tuner(gpu):

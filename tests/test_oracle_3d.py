@@ -147,6 +147,69 @@ def test_r16_offset_storage():
     assert np.array_equal(oracle.decode16(r0, oracle.STORE_BF16, 0.0), oracle.decode16(r5, oracle.STORE_BF16, 0.5))
 
 
+def _ulp16(v, store):
+    """Spacing of the 16-bit format at |v| (normal range; f16 subnormals: 2^-24)."""
+    mbits = 7 if store == oracle.STORE_BF16 else 10
+    e = np.floor(np.log2(np.maximum(np.abs(v), 1e-30)))
+    if store == oracle.STORE_F16:
+        e = np.maximum(e, -14)
+    return 2.0 ** (e - mbits)
+
+
+def _offset_cases():
+    j7 = list(K.J7_OFFSETS)
+    box = [(dh, dw, dd) for dh in (-1, 0, 1) for dw in (-1, 0, 1) for dd in (-1, 0, 1) if abs(dh) + abs(dw) + abs(dd) <= 1]
+    return [
+        ("jacobi7", K.jacobi3d()),
+        # coeff * n = 1.2: the folded offset term is 0.2 * offset
+        ("scaled_1.2", K.Stencil(K.SCALED_SUM, 3, j7, coeff=0.2, boundary=K.BOUNDARY_COPY)),
+        # every cell updated, points outside read the halo value
+        ("scaled_halo", K.Stencil(K.SCALED_SUM, 3, j7, coeff=0.15, boundary=K.BOUNDARY_HALO, halo_value=2.0)),
+        # weights summing to 0.9: the weighted path decodes u = x + offset
+        ("weighted", K.Stencil(K.WEIGHTED_SUM, 3, box, weights=[0.1, 0.15, 0.2, 0.05, 0.1, 0.2, 0.1],
+                               boundary=K.BOUNDARY_HALO, halo_value=0.5)),
+    ]
+
+
+@pytest.mark.parametrize("store", [oracle.STORE_BF16, oracle.STORE_F16])
+@pytest.mark.parametrize("offset", [0.0, 0.5, -0.75, 3.0])
+@pytest.mark.parametrize("case", range(4), ids=["jacobi7", "scaled_1.2", "scaled_halo", "weighted"])
+def test_r16_sweep_within_rounding_of_f64(store, offset, case):
+    """One 16-bit sweep, decoded (x + offset), equals the FP64 sweep of the same
+    stored state within half a 16-bit ulp of the stored result plus f32
+    rounding, whatever the offset.  Folding the offset into
+    fma(coeff, sum x_i, (coeff * n - 1) * offset) must not change the value: a
+    wrong sign or a missing term is off by about 2 * 0.2 * offset (coeff * n =
+    1.2), a halo read without the offset by 0.15 * offset, a weighted sum with
+    a folded term by 0.1 * offset."""
+    name, s = _offset_cases()[case]
+    rng = np.random.default_rng(11 + case)
+    u = rng.uniform(0.0, 1.0, (9, 10, 11)).astype(np.float32)
+    w = oracle.encode16(u, store, offset)
+    # the stored state, exactly: x + offset in f64
+    x = (oracle.decode16(w, store, 0.0)).astype(np.float64)
+    u64 = x + float(np.float32(offset))
+    want, _ = oracle.run_3d(s, u64, 1)
+    got_w, _ = oracle.run_3d_r16(s, store, w, 1, store_offset=offset)
+    got = oracle.decode16(got_w, store, 0.0).astype(np.float64) + float(np.float32(offset))
+    bound = 0.5 * _ulp16(want - offset, store) + 2.0 ** -20 * (np.abs(want) + abs(offset) + 1.0)
+    err = np.abs(got - want)
+    assert np.all(err <= bound), (name, offset, float(err.max()), float(bound[np.argmax(err - bound)]))
+
+
+def test_offset_term_closed_form():
+    """c0 = (coeff * n - 1) * offset, in double, rounded once."""
+    f = oracle.lib().oracle_scaled_sum_offset_term
+    import ctypes as C
+    f.argtypes = [C.c_float, C.c_int, C.c_float]
+    f.restype = C.c_float
+    assert f(0.2, 6, 0.5) == np.float32((float(np.float32(0.2)) * 6.0 - 1.0) * 0.5)
+    assert abs(f(0.2, 6, 0.5) - 0.1) < 1e-7
+    assert f(0.25, 4, 3.0) == 0.0
+    assert f(np.float32(1.0 / 6.0), 6, 0.5) == np.float32((float(np.float32(1.0 / 6.0)) * 6.0 - 1.0) * 0.5)
+    assert f(0.5, 6, 0.0) == 0.0
+
+
 def test_r16_stays_close_to_f64():
     u = inputs.jacobi3d_field(20, seed=9).astype(np.float64)
     want, _ = oracle.run_3d(K.jacobi3d(), u, 10)
@@ -301,7 +364,11 @@ def test_r16_stochastic_sweep_is_unbiased():
     n = 40
     u = np.ones((n, n, n), np.float32)
     u[0::2] += np.float32(2 ** -8)
-    want = oracle.run_3d(K.jacobi3d(), u, 1)[0][1:-1, 1:-1, 1:-1].astype(np.float64)
+    # the exact sweep with the f32 coefficient (in f64): 16-bit results round
+    # the deviation x = u - 1 once, at a resolution far below f32's near 1.0
+    cf = float(np.float32(1.0 / 6.0))
+    s64 = K.Stencil(K.SCALED_SUM, 3, list(K.J7_OFFSETS), coeff=cf, coeff_f64=cf, boundary=K.BOUNDARY_COPY)
+    want = oracle.run_3d(s64, u.astype(np.float64), 1)[0][1:-1, 1:-1, 1:-1]
     for store, mbits in ((oracle.STORE_BF16, 7), (oracle.STORE_F16, 10)):
         w = oracle.encode16(u, store, 1.0)
         assert np.array_equal(oracle.decode16(w, store, 1.0), u)

@@ -121,3 +121,23 @@ def test_heuristic_picks_a_candidate():
                                            for it in (1, 2, 1000)]:
         h = T.heuristic_choice(q)
         assert T.cfg_key(h) in {T.cfg_key(c) for c in T.candidates(q)}, (q, T.cfg_key(h))
+
+
+def test_packed_ensemble_is_the_members_mean():
+    """pack_params: the block-diagonal model's output equals the mean of the
+    members' outputs (FP64 forward of the oracle), for 1, 3 and 5 members and
+    two and three hidden layers."""
+    import oracle
+    from pskel_amd.tuner import pack_params
+
+    rng = np.random.default_rng(5)
+    for sizes in ([26, 64, 64, 1], [7, 5, 3, 4, 1]):
+        npar = sum(sizes[l + 1] * sizes[l] + sizes[l + 1] for l in range(len(sizes) - 1))
+        x = rng.standard_normal((40, sizes[0]))
+        for M in (1, 3, 5):
+            ps = [rng.standard_normal(npar).astype(np.float32) * 0.3 for _ in range(M)]
+            big, flat = pack_params(ps, sizes)
+            assert big == [sizes[0]] + [M * h for h in sizes[1:-1]] + [1]
+            want = np.mean([oracle.mlp_predict(sizes, p.astype(np.float64), x) for p in ps], axis=0)
+            got = oracle.mlp_predict(big, flat.astype(np.float64), x)
+            np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
