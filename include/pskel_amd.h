@@ -232,6 +232,36 @@ pskel_status_t pskel_run_iterative_host(const pskel_grid_desc_t* desc, const voi
                                         size_t iterations, size_t tile, size_t inner,
                                         const pskel_launch_cfg_t* cfg, pskel_run_stats_t* stats);
 
+/* ---- allocation-free run.  No counterpart in the reference: it is for
+ *      callers that cannot allocate device memory (serving loops, harnesses
+ *      that forbid hipMalloc).  Runs `iterations` sweeps of `stencil`, as
+ *      pskel_run_iterative does, on two caller-owned device buffers instead
+ *      of a grid; it calls no hipMalloc, hipFree or other allocating routine.
+ *      `desc` gives the dims, storage, offset, rounding, seed and device
+ *      (16-bit buffers hold the storage words of u - store_offset).  Each
+ *      buffer holds nrows rows of `pitch` elements (nrows = height in 2D,
+ *      height*width in 3D; row h (2D) or (h, w) (3D) at row*pitch); pitch must
+ *      cover a row (width in 2D, depth in 3D) and be a whole number of 1 KiB;
+ *      both buffers 256-byte aligned.  `in` is the initial state and is
+ *      overwritten with intermediate states, as runIterativeCPU overwrites
+ *      `input` (include/PSkelStencil.hpp:903-932); the result of the
+ *      iterations is left in `out` (iterations == 0 copies `in`).  2D
+ *      temporal tiles fuse an odd number of sweeps per launch (even depths
+ *      need a third buffer), which gives the same result.  Stochastic rounding
+ *      counts sweeps from 0 at every call.  Launches go to cfg->stream (NULL:
+ *      the null stream); the call returns once they are queued, or, when
+ *      stats->residuals is given, once the residuals are copied.  Residuals
+ *      use a static scratch array of each GPU (65536 FP64 block partials +
+ *      4096 results, 544 KiB, held until the process ends): at most 4096
+ *      residuals per call and launches of at most 65536 blocks with
+ *      residuals, else PSKEL_ERR_INVALID_ARGUMENT.  Calls on one GPU share
+ *      that array: calls that overlap in time on one GPU (other streams or
+ *      threads) race on it.  Not supported here: graphs (use_graph),
+ *      field-argument stencils; stats->elapsed_ms stays 0 (no events). */
+pskel_status_t pskel_run_iterative_noalloc(const pskel_grid_desc_t* desc, void* in, void* out, size_t pitch,
+                                           const pskel_stencil_desc_t* stencil, size_t iterations,
+                                           const pskel_launch_cfg_t* cfg, pskel_run_stats_t* stats);
+
 /* Out-of-core launch search: the slab height (`tile`) and sweeps per pass
  * (`inner`) for pskel_run_iterative_host.  Replaces the tiled branch of
  * runIterativeAutoGPU (include/PSkelStencil.hpp:2014-2074), whose GA picks the

@@ -184,6 +184,8 @@ def _declare(lib):
     _sig(lib, "pskel_grid_download_previous", st, P, P, I32)
     _sig(lib, "pskel_grid_info", st, P, C.POINTER(GridInfo))
     _sig(lib, "pskel_run_iterative", st, P, C.POINTER(StencilDesc), SZ, C.POINTER(LaunchCfg), C.POINTER(RunStats))
+    _sig(lib, "pskel_run_iterative_noalloc", st, C.POINTER(GridDesc), P, P, SZ, C.POINTER(StencilDesc), SZ,
+         C.POINTER(LaunchCfg), C.POINTER(RunStats))
     _sig(lib, "pskel_grid_residual", st, P, C.POINTER(D))
     _sig(lib, "pskel_grid_swap", st, P)
     _sig(lib, "pskel_grid_set_field", st, P, I32, P)

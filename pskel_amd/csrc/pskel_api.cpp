@@ -986,6 +986,140 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Allocation-free runs (pskel_run_iterative_noalloc).  The FP64 residual
+// reductions need scratch: one partial per block of a launch, then the
+// reduced residuals of the call.  It lives in a static device array, whose
+// copy on each GPU stays allocated until the process ends.  Host code reaches
+// a GPU's copy only through the address hipGetSymbolAddress returns for it
+// (resolved once per device, with that device current); the array's name is
+// not a device address on the host.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchPartials = 65536;   // partials of one launch (blocks)
+constexpr size_t kScratchResiduals = 4096;   // residuals of one call
+constexpr int kScratchDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchPartials + kScratchResiduals];
+
+namespace {
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+pskel_status_t scratch_of(int device, double** out)
+{
+    if (device < 0 || device >= kScratchDevices) return fail(PSKEL_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        PSKEL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(PSKEL_ERR_RUNTIME, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[device] = (double*)p;
+    }
+    *out = g_scratch_ptr[device];
+    return PSKEL_OK;
+}
+}  // namespace
+
+extern "C" pskel_status_t pskel_run_iterative_noalloc(const pskel_grid_desc_t* desc, void* in, void* out,
+                                                      size_t pitch, const pskel_stencil_desc_t* d,
+                                                      size_t iterations, const pskel_launch_cfg_t* cfg_in,
+                                                      pskel_run_stats_t* stats)
+{
+    if (!desc || !d || !in || !out) return fail(PSKEL_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (in == out) return fail(PSKEL_ERR_INVALID_ARGUMENT, "in and out must be distinct buffers");
+    if (stats) {
+        stats->elapsed_ms = 0.0;
+        stats->launches = 0;
+        stats->sweeps = 0;
+        stats->residual_count = 0;
+    }
+    const int stg = desc->storage;
+    if (desc->ndim != 2 && desc->ndim != 3) return fail(PSKEL_ERR_INVALID_ARGUMENT, "ndim must be 2 or 3");
+    if (stg != PSKEL_DTYPE_U8 && stg != PSKEL_DTYPE_F32 && stg != PSKEL_DTYPE_BF16 && stg != PSKEL_DTYPE_F16)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "storage must be U8, F32, BF16 or F16");
+    if (desc->ndim == 2 && (stg == PSKEL_DTYPE_BF16 || stg == PSKEL_DTYPE_F16))
+        return fail(PSKEL_ERR_UNSUPPORTED, "16-bit storage is implemented for 3D grids");
+    if (desc->ndim == 3 && stg == PSKEL_DTYPE_U8) return fail(PSKEL_ERR_UNSUPPORTED, "u8 storage is implemented for 2D grids");
+    const size_t W = desc->width, H = desc->height, D = desc->ndim == 2 ? 1 : desc->depth;
+    if (W == 0 || H == 0 || D == 0) return fail(PSKEL_ERR_INVALID_ARGUMENT, "zero-sized grid");
+    if (W > (1u << 30) || H > (1u << 30) || D > (1u << 30)) return fail(PSKEL_ERR_INVALID_ARGUMENT, "dimension too large");
+    if (desc->rounding != PSKEL_ROUND_NEAREST && desc->rounding != PSKEL_ROUND_STOCHASTIC)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "unknown rounding mode");
+    if (std::isnan(desc->store_offset))
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "the buffers hold storage words: give their storage offset");
+    if (desc->rounding == PSKEL_ROUND_STOCHASTIC && (uint64_t)W * D >= (1ull << 32))
+        return fail(PSKEL_ERR_UNSUPPORTED, "stochastic rounding needs width * depth < 2^32");
+    const int elem = elem_size(stg);
+    const size_t row = desc->ndim == 2 ? W : D;
+    if (pitch < row || (pitch * elem) % 1024 != 0)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "pitch must cover a row and be a whole number of 1 KiB");
+    if ((uintptr_t)in % 256 != 0 || (uintptr_t)out % 256 != 0)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "buffers must be 256-byte aligned");
+    pskel_launch_cfg_t cfg;
+    std::memset(&cfg, 0, sizeof(cfg));
+    if (cfg_in) cfg = *cfg_in;
+    if (cfg.use_graph) return fail(PSKEL_ERR_UNSUPPORTED, "no graphs without allocation (instantiating one allocates)");
+    if (d->kind == PSKEL_KIND_CLOUDSIM) return fail(PSKEL_ERR_UNSUPPORTED, "field arguments are attached to a grid");
+
+    // a grid over the caller's buffers: no device memory of its own, no spare
+    pskel_grid_t g;
+    g.ndim = desc->ndim;
+    g.storage = stg;
+    g.elem = elem;
+    g.W = W;
+    g.H = H;
+    g.D = D;
+    g.pitch = pitch;
+    g.nrows = desc->ndim == 2 ? H : H * W;
+    g.bytes = pitch * elem * g.nrows;
+    g.store_offset = desc->store_offset;
+    g.device = desc->device;
+    g.rounding = desc->rounding;
+    g.seed = desc->seed;
+    g.buf[0] = in;
+    g.buf[1] = out;
+    g.odd_2d_launches = true;
+    RunPlan plan;
+    pskel_status_t st = plan_run(&g, d, cfg, plan);
+    if (st != PSKEL_OK) return st;
+    const size_t every = cfg.residual_every > 0 ? (size_t)cfg.residual_every : 0;
+    const size_t nres = every ? iterations / every : 0;
+    if (nres > kScratchResiduals)
+        return fail(PSKEL_ERR_INVALID_ARGUMENT, "at most 4096 residuals per call (iterations / residual_every)");
+    PSKEL_HIP(hipSetDevice(desc->device));
+    double* scr = nullptr;
+    if ((st = scratch_of(desc->device, &scr)) != PSKEL_OK) return st;
+    if (every) {
+        const size_t nb = residual_partials(&g, plan, cfg);
+        if (nb > kScratchPartials)
+            return fail(PSKEL_ERR_INVALID_ARGUMENT, "residuals of launches of more than 65536 blocks need a grid");
+        g.d_partials = scr;
+        g.partials_cap = kScratchPartials;
+        g.d_res = scr + kScratchPartials;
+        g.res_cap = kScratchResiduals;
+    }
+    hipStream_t stream = (hipStream_t)cfg.stream;  // NULL: the null stream
+    Points pts;
+    fill_points(pts, *d, g.ndim);
+    int64_t nl = 0, nr = 0;
+    if (iterations > 0 && (st = issue(&g, *d, pts, iterations, cfg, plan, stream, &nl, &nr)) != PSKEL_OK) return st;
+    // the result goes to `out`, as runIterativeGPU leaves it in output
+    if (g.buf[0] != out) PSKEL_HIP(launch_copy(out, g.buf[0], g.bytes, stream));
+    if (stats) {
+        stats->launches = nl;
+        stats->sweeps = (int64_t)iterations;
+        stats->residual_count = nr;
+        if (nr > 0 && stats->residuals && stats->residual_capacity > 0) {
+            const size_t ncopy = (size_t)std::min<int64_t>(nr, stats->residual_capacity);
+            PSKEL_HIP(hipMemcpyAsync(stats->residuals, g.d_res, ncopy * sizeof(double), hipMemcpyDeviceToHost, stream));
+            PSKEL_HIP(hipStreamSynchronize(stream));
+        }
+    }
+    return PSKEL_OK;
+}
+
 extern "C" {
 
 pskel_status_t pskel_grid_swap(pskel_grid_t* g)

@@ -18,8 +18,8 @@ def _declared():
 
 def test_header_declares_the_entry_points():
     names = _declared()
-    for n in ("pskel_run_iterative", "pskel_grid_create", "pskel_batch_run", "pskel_tuner_choose",
-              "pskel_tuner_search", "pskel_tuner_host_search", "pskel_mlp_train"):
+    for n in ("pskel_run_iterative", "pskel_grid_create", "pskel_run_iterative_noalloc", "pskel_batch_run",
+              "pskel_tuner_choose", "pskel_mlp_train"):
         assert n in names
 
 
