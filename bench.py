@@ -50,6 +50,8 @@ WORKLOADS = {
                      label="jacobi3d-7pt 512^3 f32 x1000, FP64 residual every 50"),
     "jacobi3d_single": dict(shape=(512, 512, 512), storage="f32", sweeps=1000, residual_every=RES3D,
                             label="jacobi3d-7pt 512^3 f32 x1000, FP64 residual every 50, one sweep per launch"),
+    "jacobi3d_pair": dict(shape=(512, 512, 512), storage="f32", sweeps=1000, residual_every=RES3D,
+                          label="jacobi3d-7pt 512^3 f32 x1000, FP64 residual every 50, two sweeps per launch"),
     "jacobi3d_bf16": dict(shape=(512, 512, 512), storage="bf16", sweeps=1000, residual_every=RES3D,
                           label="jacobi3d-7pt 512^3 bf16-storage (stochastic rounding) x1000, FP64 residual every 50"),
     "jacobi3d_bf16_single": dict(shape=(512, 512, 512), storage="bf16", sweeps=1000, residual_every=RES3D,
@@ -65,13 +67,16 @@ WORKLOADS = {
                      bytes_per_cu=16),
 }
 
-# Launch shapes of the workloads the tuner does not choose for: one sweep
-# per launch (the storage comparison), CLOUDSIM (field arguments).  The other
-# tuned rows take pskel_tuner_choose's pick (--tile / --time-tile override).
+# Launch shapes of the workloads the tuner does not choose for: one and two
+# sweeps per launch (the storage comparison at equal depth), CLOUDSIM (field
+# arguments).  The other tuned rows take pskel_tuner_choose's pick (these
+# shapes are only the base of a --tile / --block / --time-tile override).
 TUNED = {
-    # two fused sweeps per launch: 18-row x 512-cell tiles of 2 x 6 waves,
-    # segment length chosen from the device's resident blocks
-    "jacobi3d": dict(tile=(4, 3, 0), block=(768, 2), time_tile=2),
+    # three fused sweeps per launch (f32): 16-row x 512-cell tiles of 2 x 4
+    # waves, segment length chosen from the device's resident blocks
+    "jacobi3d": dict(tile=(4, 4, 0), block=(512, 2), time_tile=3),
+    # two fused sweeps per launch: 18-row x 512-cell tiles of 2 x 6 waves
+    "jacobi3d_pair": dict(tile=(4, 3, 0), block=(768, 2), time_tile=2),
     "jacobi3d_bf16": dict(tile=(4, 3, 0), block=(768, 2), time_tile=2),
     "jacobi3d_f16": dict(tile=(4, 3, 0), block=(768, 2), time_tile=2),
     # one sweep per launch: the storage comparison through the same kernel
@@ -656,6 +661,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="jacobi3d", choices=sorted(WORKLOADS))
     ap.add_argument("--variant", default="tuned", choices=["tuned", "plain"])
     ap.add_argument("--tile", default="", help="tuned tile x,y,z")
+    ap.add_argument("--block", default="", help="tuned block x,y")
     ap.add_argument("--time-tile", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -686,10 +692,12 @@ def main(argv=None):
         return 0
 
     run_kw = None
-    if args.tile or args.time_tile:
+    if args.tile or args.time_tile or args.block:
         run_kw = dict(TUNED[args.workload])
         if args.tile:
             run_kw["tile"] = tuple(int(x) for x in args.tile.split(","))
+        if args.block:
+            run_kw["block"] = tuple(int(x) for x in args.block.split(","))
         if args.time_tile:
             run_kw["time_tile"] = args.time_tile
 
@@ -701,7 +709,7 @@ def main(argv=None):
 
     extra = {}
     if not args.no_extra:
-        plan = [("jacobi3d", "plain"), ("jacobi3d_single", "tuned"), ("jacobi3d_bf16", "tuned"),
+        plan = [("jacobi3d", "plain"), ("jacobi3d_single", "tuned"), ("jacobi3d_pair", "tuned"), ("jacobi3d_bf16", "tuned"),
                 ("jacobi3d_bf16_single", "tuned"), ("jacobi3d_f16", "tuned"),
                 ("life", "tuned"), ("life", "plain"), ("jacobi2d", "tuned"), ("jacobi2d", "plain"),
                 ("diffusion9", "tuned"), ("diffusion9", "plain"), ("cloudsim", "tuned"), ("cloudsim", "plain")]
@@ -751,7 +759,8 @@ def main(argv=None):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profiles(args.workload),
-        "kernel": (("k3d_march2" if head["launch"].get("time_tile", 1) == 2 else "k3d_march")
+        "kernel": ((f"k3d_march2 (depth {head['launch'].get('time_tile', 1)})"
+                    if head["launch"].get("time_tile", 1) >= 2 else "k3d_march")
                    if args.workload.startswith("jacobi3d") and args.variant == "tuned" else None),
         "sweeps_per_launch": round(head["sweeps_per_launch"], 3),
         "algorithmic_bytes_per_launch": bytes_per_launch,
@@ -806,21 +815,32 @@ def main(argv=None):
         "workloads": extra,
         "accuracy_vs_fp64": acc,
         # bf16 storage (stochastic rounding) against f32, each through its
-        # fastest path: two fused sweeps per launch for both
+        # fastest path (bf16: two fused sweeps per launch; f32: three)
         "bf16_storage_speedup_vs_f32": (round(extra["jacobi3d_bf16:tuned"]["cell_updates_per_s"] /
                                               head["cell_updates_per_s"], 3)
                                         if (args.workload, args.variant) == ("jacobi3d", "tuned")
                                         and "jacobi3d_bf16:tuned" in extra else None),
+        # storage alone at equal temporal depth: both through the fused pair
+        "bf16_storage_speedup_vs_f32_pair": (
+            round(extra["jacobi3d_bf16:tuned"]["cell_updates_per_s"] /
+                  extra["jacobi3d_pair:tuned"]["cell_updates_per_s"], 3)
+            if "jacobi3d_bf16:tuned" in extra and "jacobi3d_pair:tuned" in extra
+            and extra["jacobi3d_bf16:tuned"]["launch"].get("time_tile") == 2 else None),
         # storage alone: bf16 and f32 through the same one-sweep-per-launch kernel
         "bf16_storage_speedup_vs_f32_one_sweep_per_launch": (
             round(extra["jacobi3d_bf16_single:tuned"]["cell_updates_per_s"] /
                   extra["jacobi3d_single:tuned"]["cell_updates_per_s"], 3)
             if "jacobi3d_bf16_single:tuned" in extra and "jacobi3d_single:tuned" in extra else None),
-        # two fused sweeps per launch against one sweep per launch, f32
-        "fused_pair_speedup_vs_single_sweep": (round(head["cell_updates_per_s"] /
+        # fused sweeps against one sweep per launch, f32: the pair, and the
+        # headline's launch
+        "fused_pair_speedup_vs_single_sweep": (round(extra["jacobi3d_pair:tuned"]["cell_updates_per_s"] /
                                                      extra["jacobi3d_single:tuned"]["cell_updates_per_s"], 3)
-                                               if (args.workload, args.variant) == ("jacobi3d", "tuned")
+                                               if "jacobi3d_pair:tuned" in extra
                                                and "jacobi3d_single:tuned" in extra else None),
+        "headline_speedup_vs_single_sweep": (round(head["cell_updates_per_s"] /
+                                                   extra["jacobi3d_single:tuned"]["cell_updates_per_s"], 3)
+                                             if (args.workload, args.variant) == ("jacobi3d", "tuned")
+                                             and "jacobi3d_single:tuned" in extra else None),
         "device": {"arch": info["name"], "cus": info["cus"]},
     }
     print(json.dumps(line))

@@ -395,13 +395,19 @@ public:
 protected:
     using T = std::remove_reference_t<decltype(*std::declval<Array&>().hostArray)>;
 
-    // Sweeps the library fuses per launch at most: 2 in 3D (the 7-point
-    // Jacobi; other 3D stencils sweep singly), 4 in 2D, 8 for life.
+    // Sweeps the library fuses per launch at most: in 3D 3 for the 7-point
+    // Jacobi on f32 storage, 2 on 16-bit storage (other 3D stencils sweep
+    // singly); 4 in 2D, 8 for life.
     size_t maxTemporalDepth()
     {
         size_t kmax = dims() == 2 ? 4 : 2;
-        if constexpr (StencilTraits<Args>::has_descriptor)
-            if (dims() == 2 && StencilTraits<Args>::describe(mask, args, 2).kind == PSKEL_KIND_LIFE) kmax = 8;
+        if constexpr (StencilTraits<Args>::has_descriptor) {
+            const pskel_stencil_desc_t d = StencilTraits<Args>::describe(mask, args, dims());
+            if (dims() == 2 && d.kind == PSKEL_KIND_LIFE) kmax = 8;
+            const int storage = gpu->storage ? gpu->storage : detail::hostDtype<T>();
+            if (dims() == 3 && d.kind == PSKEL_KIND_SCALED_SUM && d.npoints == 6 && storage == PSKEL_DTYPE_F32)
+                kmax = 3;
+        }
         return kmax;
     }
 

@@ -25,7 +25,7 @@ from typing import Dict, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib as L
-from .kernels import Stencil
+from .kernels import LIFE, SCALED_SUM, Stencil
 
 _NP_TO_DTYPE = {
     np.dtype(np.uint8): L.DTYPE_U8,
@@ -458,8 +458,15 @@ class _StencilBase:
                               innerIterations: int = 1, GPUBlockSizeX: int = 0, GPUBlockSizeY: int = 0):
         """Tiled run (include/PSkelStencil.hpp:1663-1760).  The grid is resident, so
         tiles are LDS tiles: innerIterations is the temporal depth (sweeps per
-        launch, capped at 4); the result is identical to runIterativeGPU."""
-        k = max(1, min(int(innerIterations), 4))
+        launch, capped at 4 in 2D, 8 for life, 3 in 3D for the f32 7-point
+        Jacobi and 2 for other 3D grids); the result is identical to
+        runIterativeGPU."""
+        st = self._stencil()
+        if self.ndim == 3:
+            cap = 3 if self._storage() == "f32" and st.kind == SCALED_SUM and len(st.offsets) == 6 else 2
+        else:
+            cap = 8 if st.kind == LIFE else 4
+        k = max(1, min(int(innerIterations), cap))
         return self._run(iterations, variant="tuned", time_tile=k)
 
     def runTilingGPU(self, tilingWidth: int, tilingHeight: int, tilingDepth: int, GPUBlockSizeX: int = 0,

@@ -1,23 +1,27 @@
-// Two fused sweeps per launch (temporal depth 2) of the 7-point scaled sum
-// with COPY faces, f32, bf16 or f16 storage.
+// DEPTH fused sweeps per launch (temporal depth 2 or 3) of the 7-point
+// scaled sum with COPY faces, f32, bf16 or f16 storage.
 //
 // A block owns an extended (w, d) tile and marches along h.  At plane h it
 // computes the first sweep s1(h) on every cell of the tile from the input
-// planes h-1, h, h+1, then the second sweep s2(h-1) from s1(h-2), s1(h-1),
-// s1(h) on the cells whose neighbours all lie in the tile, and on grid faces,
-// which copy.  Every cell of both sweeps evaluates
+// planes h-1, h, h+1, then each later sweep l one plane behind the one
+// before, s_l(h-l+1) from s_(l-1)(h-l), s_(l-1)(h-l+1), s_(l-1)(h-l+2), on the
+// cells whose neighbours all lie in the tile, and on grid faces, which copy.
+// The last sweep's plane h-DEPTH+1 is stored.  Every cell of every sweep
+// evaluates
 //     coeff * (((((h- + h+) + w-) + w+) + d-) + d+)
-// in f32 with no contraction, so a fused pair equals two single sweeps bit for
-// bit.  Each segment of h planes recomputes s1 on the plane before and after
-// it.
+// in f32 with no contraction, so a fused launch equals DEPTH single sweeps bit
+// for bit.  Each segment of h planes recomputes the earlier sweeps on the
+// DEPTH-1 planes before and after it.
 //
 // Tile.  The block's waves form NWW rows of NWD waves.  A wave holds RW rows
 // of 64 lanes x VD cells, so the tile is TW = NWW*RW rows by TD = NWD*64*VD
 // cells.  Along w the two rows just outside the tile are loaded as a halo
-// (by the top and bottom row of waves), s1 is exact on all TW rows, and the
-// tile outputs the TW-2 rows inside; neighbouring tiles overlap by two rows.
-// Along d no halo is loaded: tiles overlap by one lane vector on each side.
-// Each output cell is written by exactly one tile.
+// (by the top and bottom row of waves), s1 is exact on all TW rows, each later
+// sweep on one row fewer at each edge, and the tile outputs the TW-2(DEPTH-1)
+// rows inside; neighbouring tiles overlap by 2(DEPTH-1) rows.  Along d no
+// halo is loaded: tiles overlap by one lane vector on each side, which covers
+// the DEPTH-1 cells each edge loses (DEPTH <= VD).  Each output cell is
+// written by exactly one tile.
 //
 // Exchange.  Planes along h sit in registers.  A wave's interior rows see
 // their w neighbours in its own registers; only its top and bottom rows go
@@ -51,6 +55,7 @@ namespace pskel {
 
 struct March2Shape {
     int vd, rw, nwd, nww, pf;
+    int depth;     // fused sweeps per launch (2 or 3)
     int hseg;      // h planes per block; 0: chosen to fill the blocks the device holds at once
     int ntd, ntw;  // tiles along d and w
 };
@@ -60,33 +65,40 @@ inline int ceil_div_i(int a, int b) { return (a + b - 1) / b; }
 // cfg.tile_x = cells per lane (VD), tile_y = rows per wave (RW), tile_z = h
 // planes per block (0: chosen from the device's resident blocks), block_x =
 // threads per block (64 * NWD * NWW), block_y = waves along d (NWD) + 16 *
-// (planes prefetched beyond h+1, minus 1)
-inline March2Shape march2_shape(const Sweep3D& s, const pskel_launch_cfg_t& cfg)
+// (planes prefetched beyond h+1, minus 1); depth = sweeps of the launch
+inline March2Shape march2_shape(const Sweep3D& s, const pskel_launch_cfg_t& cfg, int depth = 2)
 {
     March2Shape m;
-    // default: 18-row tiles of 2 x 6 waves (16 rows advanced per tile, so a
-    // 512-wide grid takes 32 tiles and 8 segments fill the 256 CUs of an MI355X)
+    m.depth = depth;
+    // default at depth 2: 18-row tiles of 2 x 6 waves (16 rows advanced per
+    // tile, so a 512-wide grid takes 32 tiles and 8 segments fill the 256 CUs
+    // of an MI355X); at depth 3: 16-row tiles of 2 x 4 waves of 4 rows
     m.vd = cfg.tile_x > 0 ? cfg.tile_x : 4;
-    m.rw = cfg.tile_y > 0 ? cfg.tile_y : 3;
-    const int nw = cfg.block_x > 0 ? cfg.block_x / kWave : 12;
+    m.rw = cfg.tile_y > 0 ? cfg.tile_y : (depth == 3 ? 4 : 3);
+    const int nw = cfg.block_x > 0 ? cfg.block_x / kWave : (depth == 3 ? 8 : 12);
     m.nwd = cfg.block_y > 0 ? cfg.block_y % 16 : 2;
     m.pf = cfg.block_y > 0 ? 1 + cfg.block_y / 16 : 1;
     m.nww = m.nwd > 0 ? nw / m.nwd : 0;
     m.hseg = cfg.tile_z > 0 ? cfg.tile_z : 0;
     const int TD = m.nwd * kWave * m.vd, TW = m.nww * m.rw;
     m.ntd = s.D <= TD ? 1 : 1 + ceil_div_i(s.D - TD, TD - 2 * m.vd);
-    m.ntw = s.W <= TW ? 1 : 1 + ceil_div_i(s.W - TW, TW - 2);
+    m.ntw = s.W <= TW ? 1 : 1 + ceil_div_i(s.W - TW, TW - 2 * (depth - 1));
     return m;
 }
 
-// the instantiated shapes (vd, rw, nwd, nww, pf): ten for f32, four for 16-bit
-// storage (rows of 512, 256 and 128 cells along d)
+// the instantiated shapes (vd, rw, nwd, nww, pf): at depth 2 ten for f32 and
+// four for 16-bit storage (rows of 512, 256 and 128 cells along d); at depth 3
+// one for f32 (16-row tiles of 512 cells: 2 x 4 waves of 4 rows x 64 lanes x 4
+// cells; the 12-wave shapes spill at depth 3)
 inline bool march2_supported(const March2Shape& m, int storage)
 {
     auto key = [](int vd, int rw, int nwd, int nww, int pf) {
         return ((((vd * 16 + rw) * 16 + nwd) * 16) + nww) * 16 + pf;
     };
     const int k = key(m.vd, m.rw, m.nwd, m.nww, m.pf);
+    if (m.depth == 3)  // 8 waves: three register rings of 4 rows x 4 cells fit 2 waves per SIMD
+        return storage == PSKEL_DTYPE_F32 && k == key(4, 4, 2, 4, 1);
+    if (m.depth != 2) return false;
     if (storage != PSKEL_DTYPE_F32)
         return k == key(4, 3, 2, 6, 1) || k == key(4, 2, 2, 8, 1) || k == key(4, 3, 1, 12, 1) ||
                k == key(2, 3, 1, 12, 1);
@@ -266,7 +278,7 @@ __device__ __forceinline__ float from_upper_lane(float v, float edge)
                                                       0xf, false));
 }
 
-template <int ST, int VD, int RW, int NWD, int NWW, int PF, bool RES, bool SR>
+template <int ST, int VD, int RW, int NWD, int NWW, int PF, int DEPTH, bool RES, bool SR>
 __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float coeff, int hseg, int ntd, int ntw)
 {
     using T = typename Stor<ST>::T;  // storage element
@@ -277,13 +289,15 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
     constexpr int TD = NWD * kWave * VD;
     constexpr int TW = NWW * RW;
     constexpr int NL = NWD * kWave;  // LDS words per row
+    constexpr int NS = DEPTH - 1;  // sweeps kept in register rings (all but the last)
     static_assert(RW >= 2 && NWW >= 2 && (VD == 2 || VD == 4), "fused tile shape");
+    static_assert(DEPTH >= 2 && DEPTH <= VD, "tiles overlap by one lane vector along d");
     using LV = typename LVec<VD>::T;
     // input plane h: each row of waves' top and bottom row, then the halo rows above and below the tile
     __shared__ LV lin[2][2 * NWW + 2][NL];
-    __shared__ LV ls1[2][2 * NWW][NL];        // s1(h-1): each row of waves' top and bottom row
-    __shared__ float lein[2][NWW][NWD][RW][2];  // first and last cell of each wave's rows, input plane h
-    __shared__ float les1[2][NWW][NWD][RW][2];  // the same for s1(h-1)
+    __shared__ LV ls[NS][2][2 * NWW][NL];         // s_l(h-l): each row of waves' top and bottom row
+    __shared__ float lein[2][NWW][NWD][RW][2];    // first and last cell of each wave's rows, input plane h
+    __shared__ float les[NS][2][NWW][NWD][RW][2];  // the same for s_l(h-l)
     __shared__ double red[NW];
 
     const int lane = threadIdx.x % kWave;
@@ -298,9 +312,10 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
     // 16-bit storage: the stored values x = u - off are summed as they are and
     // the result is fma(coeff, sum, c0) (scaled_sum_offset_term)
     const float c0 = s.c0;
-    // stochastic rounding: the keys of the launch's two sweeps
-    const uint32_t key1 = SR ? sweep_key(s.seed, s.sweep) : 0u;
-    const uint32_t key2 = SR ? sweep_key(s.seed, s.sweep + 1) : 0u;
+    // stochastic rounding: the keys of the launch's sweeps
+    uint32_t keys[DEPTH];
+#pragma unroll
+    for (int l = 0; l < DEPTH; ++l) keys[l] = SR ? sweep_key(s.seed, s.sweep + l) : 0u;
     double rr = 0.0;
 
     // one segment: planes [h0, h1) of tile (td, tw)
@@ -309,16 +324,18 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         const int ed0 = td * (TD - 2 * VD);
         const int od0 = td == 0 ? 0 : ed0 + VD;
         const int od1 = ed0 + TD >= D ? D : ed0 + TD - VD;
-        const int ew0 = tw * (TW - 2);
-        const int ow0 = tw == 0 ? 0 : ew0 + 1;
-        const int ow1 = ew0 + TW >= W ? W : ew0 + TW - 1;
+        const int ew0 = tw * (TW - 2 * NS);
+        const int ow0 = tw == 0 ? 0 : ew0 + NS;
+        const int ow1 = ew0 + TW >= W ? W : ew0 + TW - NS;
 
         const int col = wd * kWave + lane;  // LDS word of this lane in a row
         const int d = ed0 + col * VD;
         const int wr = ew0 + ww * RW;
-        const int hlo = max(h0 - 2, 0), hhi = min(h1 + 2, H);  // input planes read
-        const int ha = max(h0 - 1, 0), hb = min(h1 + 1, H);    // first-sweep planes
-        const int hend = hb == H ? hb : hb - 1;                  // a segment at the grid's end drains one more step
+        const int hlo = max(h0 - DEPTH, 0), hhi = min(h1 + DEPTH, H);  // input planes read
+        // sweep l (1-based) runs on planes [h0 - (DEPTH - l), h1 + (DEPTH - l)) within the grid;
+        // step h computes its plane h - l + 1, and the last sweep's plane h1 - 1 at step h1 + DEPTH - 2
+        const int ha = max(h0 - NS, 0);
+        const int hend = h1 + DEPTH - 2;
 
         const bool dlive = d < D;
         const bool dout = dlive && d >= od0 && d < od1;
@@ -374,8 +391,8 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         Word in[NQ][RW][NV];        // input planes as loaded (storage words)
         Word hal[NQ][NV];           // input planes of the halo row
         float fin[NQ][RW][VD];      // input planes h-1, h, h+1, decoded once each
-        Word s1[NQ][RW][NV];        // first-sweep planes, storage words
-        float fs1[NQ][RW][VD];      // first-sweep planes h-2, h-1, h, decoded once each
+        Word sw[NS][NQ][RW][NV];    // sweep l+1's planes, storage words
+        float fs[NS][NQ][RW][VD];   // sweep l+1's planes h-l-2, h-l-1, h-l, decoded once each
 
         auto load = [&](int p, Word (&v)[RW][NV], Word (&hv)[NV]) {
             const bool ok = p >= hlo && p < hhi;  // uniform across the block
@@ -472,12 +489,14 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         };
 
 #pragma unroll
-        for (int i = 0; i < RW; ++i) {
+        for (int l = 0; l < NS; ++l)
 #pragma unroll
-            for (int k = 0; k < NV; ++k) s1[0][i][k] = s1[1][i][k] = Word(0);
+            for (int i = 0; i < RW; ++i) {
 #pragma unroll
-            for (int j = 0; j < VD; ++j) fs1[0][i][j] = fs1[1][i][j] = 0.f;
-        }
+                for (int k = 0; k < NV; ++k) sw[l][0][i][k] = sw[l][1][i][k] = Word(0);
+#pragma unroll
+                for (int j = 0; j < VD; ++j) fs[l][0][i][j] = fs[l][1][i][j] = 0.f;
+            }
         // prologue: each load followed by discarded stores, the order of the
         // steady state, so the wait counts the compiler derives at the loop head
         // hold on every path into it
@@ -487,7 +506,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
             for (int k = 0; k < PF + 2; ++k) {
                 load(ha - 1 + k, in[k], hal[k]);
 #pragma unroll
-                for (int i = 0; i < RW; ++i) buf_store<Word, NV>(none, kNoAccess, s1[0][i]);
+                for (int i = 0; i < RW; ++i) buf_store<Word, NV>(none, kNoAccess, sw[0][0][i]);
             }
         }
         // planes ha-1 and ha, decoded (the steady state decodes plane h+1)
@@ -495,14 +514,22 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         decode(in[1], fin[1]);
 
         int b = 0;
+        auto clear = [&](Word (&w)[RW][NV], float (&f)[RW][VD]) {
+#pragma unroll
+            for (int i = 0; i < RW; ++i) {
+#pragma unroll
+                for (int k = 0; k < NV; ++k) w[i][k] = Word(0);
+#pragma unroll
+                for (int j = 0; j < VD; ++j) f[i][j] = 0.f;
+            }
+        };
         // step h at ring phase S: input slot (S + k) % NQ holds plane h - 1 + k,
-        // s1 slot (S + k) % NQ the first sweep of plane h - 2 + k
+        // slot (S + k) % NQ of sweep l+1's ring (l = 0 .. NS-1) its plane h - l - 2 + k
         auto step = [&](auto phase, int h) __attribute__((always_inline)) {
             constexpr int S = decltype(phase)::value;
             constexpr int I0 = S % NQ, I1 = (S + 1) % NQ, I2 = (S + 2) % NQ, IF = (S + NQ - 1) % NQ;
             load(h + PF + 1, in[IF], hal[IF]);
-            const int p2 = h - 1;
-            // plane h and s1(h-1) into LDS
+            // plane h and s_(l+1)(h-l-1) into LDS
             stage(lin[b], lein[b], fin[I1]);
             if (top || bottom) {
                 float fh[VD];
@@ -510,25 +537,38 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
                 for (int j = 0; j < VD; ++j) fh[j] = LW::raw(hal[I1], j);
                 lds_put<VD>(&lin[b][hslot][col], fh);
             }
-            stage(ls1[b], les1[b], fs1[I1]);
+#pragma unroll
+            for (int l = 0; l < NS; ++l) stage(ls[l][b], les[l][b], fs[l][I1]);
             __syncthreads();
             decode(in[I2], fin[I2]);  // plane h+1
-            if (h < hb) {
-                sweep(std::true_type{}, h, fin[I0], fin[I1], fin[I2], in[I1], lin[b], lein[b], key1, s1[I2]);
-                decode(s1[I2], fs1[I2]);
+            // sweep 1 at plane h (h >= ha: the loop starts there)
+            if (h < min(h1 + NS, H)) {
+                sweep(std::true_type{}, h, fin[I0], fin[I1], fin[I2], in[I1], lin[b], lein[b], keys[0], sw[0][I2]);
+                decode(sw[0][I2], fs[0][I2]);
             } else {
-#pragma unroll
-                for (int i = 0; i < RW; ++i) {
-#pragma unroll
-                    for (int k = 0; k < NV; ++k) s1[I2][i][k] = Word(0);
-#pragma unroll
-                    for (int j = 0; j < VD; ++j) fs1[I2][i][j] = 0.f;
-                }
+                clear(sw[0][I2], fs[0][I2]);
             }
-            const bool out2 = p2 >= h0 && p2 < h1;  // uniform across the block
+            // sweeps 2 .. DEPTH-1 at planes h-1 .. h-NS+1 (uniform bounds)
+            unroll_seq([&](auto lc) __attribute__((always_inline)) {
+                constexpr int l = decltype(lc)::value + 1;
+                if constexpr (l < NS) {
+                    const int q = h - l;
+                    if (q >= max(h0 - (NS - l), 0) && q < min(h1 + (NS - l), H)) {
+                        sweep(std::false_type{}, q, fs[l - 1][I0], fs[l - 1][I1], fs[l - 1][I2], sw[l - 1][I1],
+                              ls[l - 1][b], les[l - 1][b], keys[l], sw[l][I2]);
+                        decode(sw[l][I2], fs[l][I2]);
+                    } else {
+                        clear(sw[l][I2], fs[l][I2]);
+                    }
+                }
+            }, std::make_integer_sequence<int, NS>{});
+            // the last sweep at plane h - NS, stored
+            const int pl = h - NS;
+            const bool out2 = pl >= h0 && pl < h1;  // uniform across the block
             Word o[RW][NV];
             if (out2) {
-                sweep(std::false_type{}, p2, fs1[I0], fs1[I1], fs1[I2], s1[I1], ls1[b], les1[b], key2, o);
+                sweep(std::false_type{}, pl, fs[NS - 1][I0], fs[NS - 1][I1], fs[NS - 1][I2], sw[NS - 1][I1],
+                      ls[NS - 1][b], les[NS - 1][b], keys[NS], o);
                 if constexpr (RES) {
 #pragma unroll
                     for (int i = 0; i < RW; ++i) {
@@ -536,7 +576,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
                         for (int j = 0; j < VD; ++j) {
                             if (d + j < D) {
-                                const double dv = (double)LW::raw(o[i], j) - (double)fs1[I1][i][j];
+                                const double dv = (double)LW::raw(o[i], j) - (double)fs[NS - 1][I1][i][j];
                                 rr += dv * dv;
                             }
                         }
@@ -548,7 +588,7 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 #pragma unroll
                     for (int k = 0; k < NV; ++k) o[i][k] = Word(0);
             }
-            const auto rs = plane_rsrc((T*)s.dst + (size_t)(out2 ? p2 : 0) * plane, out2 ? plane_bytes : 0);
+            const auto rs = plane_rsrc((T*)s.dst + (size_t)(out2 ? pl : 0) * plane, out2 ? plane_bytes : 0);
 #pragma unroll
             for (int i = 0; i < RW; ++i) buf_store<Word, NV>(rs, st_off[i], o[i]);
             b ^= 1;
@@ -578,9 +618,9 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
 // Blocks and h planes per block.  cfg.tile_z fixes the planes; by default
 // the segment count n maximises the useful fraction of the device's block
 // slots (CUs x resident blocks per CU) over the rounds the grid takes, net of
-// the 4 planes a segment reads twice: (tiles * n / (rounds * slots)) *
-// (hseg / (hseg + 4)).
-template <int ST, int VD, int RW, int NWD, int NWW, int PF>
+// the 2*DEPTH planes a segment reads twice: (tiles * n / (rounds * slots)) *
+// (hseg / (hseg + 2*DEPTH)).
+template <int ST, int VD, int RW, int NWD, int NWW, int PF, int DEPTH>
 size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
 {
     const int64_t tiles = (int64_t)m.ntd * m.ntw;
@@ -594,15 +634,15 @@ size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
             constexpr int NT = NWD * NWW * kWave;
             int o[4] = {1, 1, 1, 1};
             bool ok = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                          &o[0], k3d_march2<ST, VD, RW, NWD, NWW, PF, true, false>, NT, 0) == hipSuccess &&
+                          &o[0], k3d_march2<ST, VD, RW, NWD, NWW, PF, DEPTH, true, false>, NT, 0) == hipSuccess &&
                       hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                          &o[1], k3d_march2<ST, VD, RW, NWD, NWW, PF, false, false>, NT, 0) == hipSuccess;
+                          &o[1], k3d_march2<ST, VD, RW, NWD, NWW, PF, DEPTH, false, false>, NT, 0) == hipSuccess;
             if constexpr (ST != PSKEL_DTYPE_F32)
                 ok = ok &&
                      hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &o[2], k3d_march2<ST, VD, RW, NWD, NWW, PF, true, true>, NT, 0) == hipSuccess &&
+                         &o[2], k3d_march2<ST, VD, RW, NWD, NWW, PF, DEPTH, true, true>, NT, 0) == hipSuccess &&
                      hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &o[3], k3d_march2<ST, VD, RW, NWD, NWW, PF, false, true>, NT, 0) == hipSuccess;
+                         &o[3], k3d_march2<ST, VD, RW, NWD, NWW, PF, DEPTH, false, true>, NT, 0) == hipSuccess;
             occ = ok ? std::max(1, std::min(std::min(o[0], o[1]), std::min(o[2], o[3]))) : 1;
             occ_cache.store(occ, std::memory_order_relaxed);
         }
@@ -621,7 +661,7 @@ size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
             const int hs = (s.H + n - 1) / n;
             const int64_t nb = tiles * ((s.H + hs - 1) / hs);
             const int64_t rounds = (nb + slots - 1) / slots;
-            const double u = (double)nb / (double)(rounds * slots) * hs / (hs + 4.0);
+            const double u = (double)nb / (double)(rounds * slots) * hs / (hs + 2.0 * DEPTH);
             if (u > best + 1e-9) {
                 best = u;
                 q = hs;
@@ -632,15 +672,16 @@ size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
     return (size_t)(tiles * ((s.H + q - 1) / q));
 }
 
-template <int ST, int VD, int RW, int NWD, int NWW, int PF>
+template <int ST, int VD, int RW, int NWD, int NWW, int PF, int DEPTH>
 hipError_t march2_launch(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st)
 {
     int hseg = 0;
-    const dim3 grid((unsigned)march2_grid<ST, VD, RW, NWD, NWW, PF>(s, m, &hseg)), block(NWD * NWW * kWave);
+    const dim3 grid((unsigned)march2_grid<ST, VD, RW, NWD, NWW, PF, DEPTH>(s, m, &hseg)),
+        block(NWD * NWW * kWave);
     const bool sr = ST != PSKEL_DTYPE_F32 && s.rounding == PSKEL_ROUND_STOCHASTIC;
-#define PSKEL_MARCH2_GO(RESV, SRV)                                                                           \
-    hipLaunchKernelGGL((k3d_march2<ST, VD, RW, NWD, NWW, PF, RESV, SRV>), grid, block, 0, st, s, p.coeff, hseg, \
-                       m.ntd, m.ntw)
+#define PSKEL_MARCH2_GO(RESV, SRV)                                                                     \
+    hipLaunchKernelGGL((k3d_march2<ST, VD, RW, NWD, NWW, PF, DEPTH, RESV, SRV>), grid, block, 0, st, s, \
+                       p.coeff, hseg, m.ntd, m.ntw)
     if constexpr (ST == PSKEL_DTYPE_F32) {
         if (s.partials) PSKEL_MARCH2_GO(true, false);
         else PSKEL_MARCH2_GO(false, false);
@@ -654,31 +695,32 @@ hipError_t march2_launch(const Sweep3D& s, const Points& p, const March2Shape& m
     return hipGetLastError();
 }
 
-// instantiated shapes (vd, rw, nwd, nww, pf), as march2_supported lists them
+// instantiated shapes (vd, rw, nwd, nww, pf, depth), as march2_supported lists them
 #define PSKEL_MARCH2_SHAPES_F32(X) \
-    X(2, 4, 4, 4, 1)               \
-    X(2, 4, 4, 2, 1)               \
-    X(4, 2, 2, 8, 1)               \
-    X(4, 2, 2, 4, 1)               \
-    X(2, 4, 4, 4, 2)               \
-    X(2, 4, 2, 4, 1)               \
-    X(4, 4, 2, 4, 1)               \
-    X(4, 3, 2, 6, 1)               \
-    X(4, 3, 1, 12, 1)              \
-    X(2, 3, 1, 12, 1)
+    X(2, 4, 4, 4, 1, 2)            \
+    X(2, 4, 4, 2, 1, 2)            \
+    X(4, 2, 2, 8, 1, 2)            \
+    X(4, 2, 2, 4, 1, 2)            \
+    X(2, 4, 4, 4, 2, 2)            \
+    X(2, 4, 2, 4, 1, 2)            \
+    X(4, 4, 2, 4, 1, 2)            \
+    X(4, 3, 2, 6, 1, 2)            \
+    X(4, 3, 1, 12, 1, 2)           \
+    X(2, 3, 1, 12, 1, 2)           \
+    X(4, 4, 2, 4, 1, 3)
 #define PSKEL_MARCH2_SHAPES_16(X) \
-    X(4, 3, 2, 6, 1)              \
-    X(4, 2, 2, 8, 1)              \
-    X(4, 3, 1, 12, 1)             \
-    X(2, 3, 1, 12, 1)
+    X(4, 3, 2, 6, 1, 2)           \
+    X(4, 2, 2, 8, 1, 2)           \
+    X(4, 3, 1, 12, 1, 2)          \
+    X(2, 3, 1, 12, 1, 2)
 
 template <int ST>
 size_t march2_blocks(const Sweep3D& s, const March2Shape& m)
 {
     int hseg = 0;
-#define PSKEL_MARCH2(VD, RW, NWD, NWW, PF)                                       \
-    if (m.vd == VD && m.rw == RW && m.nwd == NWD && m.nww == NWW && m.pf == PF) \
-        return march2_grid<ST, VD, RW, NWD, NWW, PF>(s, m, &hseg);
+#define PSKEL_MARCH2(VD, RW, NWD, NWW, PF, DEPTH)                                                         \
+    if (m.vd == VD && m.rw == RW && m.nwd == NWD && m.nww == NWW && m.pf == PF && m.depth == DEPTH) \
+        return march2_grid<ST, VD, RW, NWD, NWW, PF, DEPTH>(s, m, &hseg);
     if constexpr (ST == PSKEL_DTYPE_F32) {
         PSKEL_MARCH2_SHAPES_F32(PSKEL_MARCH2)
     } else {
@@ -692,9 +734,9 @@ template <int ST>
 hipError_t march2_dispatch(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st)
 {
     if (!march2_supported(m, ST)) return hipErrorInvalidValue;
-#define PSKEL_MARCH2(VD, RW, NWD, NWW, PF)                                       \
-    if (m.vd == VD && m.rw == RW && m.nwd == NWD && m.nww == NWW && m.pf == PF) \
-        return march2_launch<ST, VD, RW, NWD, NWW, PF>(s, p, m, st);
+#define PSKEL_MARCH2(VD, RW, NWD, NWW, PF, DEPTH)                                                         \
+    if (m.vd == VD && m.rw == RW && m.nwd == NWD && m.nww == NWW && m.pf == PF && m.depth == DEPTH) \
+        return march2_launch<ST, VD, RW, NWD, NWW, PF, DEPTH>(s, p, m, st);
     if constexpr (ST == PSKEL_DTYPE_F32) {
         PSKEL_MARCH2_SHAPES_F32(PSKEL_MARCH2)
     } else {

@@ -176,9 +176,9 @@ hipError_t launch_3d_tuned(const Sweep3D& s, const Points& p, int storage,
     }
 }
 
-size_t ttile3d_blocks(const Sweep3D& s, int storage, const pskel_launch_cfg_t& cfg)
+size_t ttile3d_blocks(const Sweep3D& s, int storage, const pskel_launch_cfg_t& cfg, int depth)
 {
-    const March2Shape m = march2_shape(s, cfg);
+    const March2Shape m = march2_shape(s, cfg, depth);
     switch (storage) {
     case PSKEL_DTYPE_F32: return march3d_t2_f32_blocks(s, m);
     case PSKEL_DTYPE_BF16: return march3d_t2_bf16_blocks(s, m);
@@ -187,20 +187,20 @@ size_t ttile3d_blocks(const Sweep3D& s, int storage, const pskel_launch_cfg_t& c
     }
 }
 
-bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg, int storage)
+bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg, int storage, int depth)
 {
     if (cfg.block_x > 0 && cfg.block_x % kWave != 0) return false;
     Sweep3D s{};
     s.W = s.H = s.D = 1;
-    const March2Shape m = march2_shape(s, cfg);
-    return m.nwd > 0 && m.nww > 0 && m.nwd * m.nww * kWave == (cfg.block_x > 0 ? cfg.block_x : 12 * kWave) &&
-           march2_supported(m, storage);
+    const March2Shape m = march2_shape(s, cfg, depth);
+    const int threads = cfg.block_x > 0 ? cfg.block_x : (depth == 3 ? 8 : 12) * kWave;
+    return m.nwd > 0 && m.nww > 0 && m.nwd * m.nww * kWave == threads && march2_supported(m, storage);
 }
 
-hipError_t launch_3d_ttile(const Sweep3D& s, const Points& p, int storage, const pskel_launch_cfg_t& cfg,
+hipError_t launch_3d_ttile(const Sweep3D& s, const Points& p, int storage, const pskel_launch_cfg_t& cfg, int depth,
                            hipStream_t st)
 {
-    const March2Shape m = march2_shape(s, cfg);
+    const March2Shape m = march2_shape(s, cfg, depth);
     switch (storage) {
     case PSKEL_DTYPE_F32: return march3d_t2_f32(s, p, m, st);
     case PSKEL_DTYPE_BF16: return march3d_t2_bf16(s, p, m, st);

@@ -422,9 +422,9 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
             if (g->storage == PSKEL_DTYPE_BF16 || g->storage == PSKEL_DTYPE_F16)
                 s.c0 = scaled_sum_offset_term(pts.coeff, pts.np, g->store_offset);
             size_t nb;
-            if (k == 2) {
-                nb = ttile3d_blocks(s, g->storage, cfg);
-                e = launch_3d_ttile(s, pts, g->storage, cfg, st);
+            if (k >= 2) {
+                nb = ttile3d_blocks(s, g->storage, cfg, (int)k);
+                e = launch_3d_ttile(s, pts, g->storage, cfg, (int)k, st);
             } else if (plan.path >= 1) {
                 nb = tuned3d_blocks(s, g->storage, one);
                 e = launch_3d_tuned(s, pts, g->storage, one, st);
@@ -530,16 +530,18 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
             // one generation per launch.
             plan.path = (plan.pattern == PAT_LIFE8 && radius != 1) ? 1 : 2;
         }
-        if (cfg.time_tile > 2 && ndim == 3) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 2");
-        // two fused sweeps per launch run the 7-point Jacobi; for any other 3D
-        // stencil the depth is a hint and sweeps run singly
-        if (ndim == 3 && cfg.time_tile == 2 && tunable) {
+        if (cfg.time_tile > 3 && ndim == 3) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 3");
+        // two or three fused sweeps per launch run the 7-point Jacobi; for any
+        // other 3D stencil the depth is a hint and sweeps run singly
+        if (ndim == 3 && cfg.time_tile >= 2 && tunable) {
             // the fused kernel addresses a plane through a buffer of < 2^31 bytes
             if ((uint64_t)g->W * g->pitch * g->elem > 0x7fffffffull)
                 return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal tiles need planes under 2 GiB");
-            if (!ttile3d_shape_ok(cfg, g->storage))
+            // a run of depth 3 ends with a pair when its length or a residual
+            // interval leaves two sweeps: the shape must exist at both depths
+            if (!ttile3d_shape_ok(cfg, g->storage, cfg.time_tile) || !ttile3d_shape_ok(cfg, g->storage, 2))
                 return fail(PSKEL_ERR_UNSUPPORTED,
-                            "3D temporal tile shape (cells per lane, rows per wave, waves along d and w)");
+                            "3D temporal tile shape (cells per lane, rows per wave, waves along d and w, depth)");
             plan.path = 2;
         }
         if (!tunable) plan.path = 0;  // no specialised kernel for this point set: the plain kernel runs
@@ -569,7 +571,9 @@ size_t residual_partials(const pskel_grid_t* g, const RunPlan& plan, const pskel
         pskel_launch_cfg_t one = cfg;
         if (plan.path == 2) one.tile_x = one.tile_y = one.tile_z = one.block_x = one.block_y = 0;
         nb = std::max(nb, plan.path >= 1 ? tuned3d_blocks(s3, g->storage, one) : plain3d_blocks(s3, bx, by));
-        if (plan.path == 2) nb = std::max(nb, ttile3d_blocks(s3, g->storage, cfg));
+        if (plan.path == 2)
+            for (int k = 2; k <= std::max(2, cfg.time_tile); ++k)
+                nb = std::max(nb, ttile3d_blocks(s3, g->storage, cfg, k));
     }
     return nb;
 }
@@ -883,7 +887,7 @@ pskel_status_t pskel_run_iterative(pskel_grid_t* g, const pskel_stencil_desc_t* 
         // the fused kernel sizes its grid from device queries: make them here,
         // outside any stream capture below
         Sweep3D s3 = make_sweep3d(g, nullptr, nullptr, radius, nullptr);
-        (void)ttile3d_blocks(s3, g->storage, cfg);
+        for (int k = 2; k <= std::max(2, cfg.time_tile); ++k) (void)ttile3d_blocks(s3, g->storage, cfg, k);
     }
     if (plan.path == 2 && ndim == 2 && !g->buf[2]) {
         PSKEL_HIP(hipMalloc(&g->buf[2], g->bytes));

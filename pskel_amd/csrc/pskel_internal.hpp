@@ -176,10 +176,12 @@ size_t plain3d_blocks(const Sweep3D& s, int bx, int by);
 size_t tuned3d_blocks(const Sweep3D& s, int storage, const pskel_launch_cfg_t& cfg);
 // two fused sweeps per launch (7-point scaled sum, any storage); cfg's tile
 // fields describe the fused kernel's tile
-hipError_t launch_3d_ttile(const Sweep3D& s, const Points& p, int storage, const pskel_launch_cfg_t& cfg,
+// `depth` fused sweeps (2 or 3) of the 7-point Jacobi per launch
+hipError_t launch_3d_ttile(const Sweep3D& s, const Points& p, int storage, const pskel_launch_cfg_t& cfg, int depth,
                            hipStream_t st);
-size_t ttile3d_blocks(const Sweep3D& s, int storage, const pskel_launch_cfg_t& cfg);
-bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg, int storage);  // an instantiated fused tile shape
+size_t ttile3d_blocks(const Sweep3D& s, int storage, const pskel_launch_cfg_t& cfg, int depth);
+// an instantiated fused tile shape at that depth
+bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg, int storage, int depth);
 
 // Deterministic second-stage reduction: out = sqrt(sum(partials[0..n))).
 hipError_t launch_reduce_partials(const double* partials, size_t n, double* out, hipStream_t st);

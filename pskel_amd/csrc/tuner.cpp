@@ -74,11 +74,11 @@ pskel_launch_cfg_t tuned(int tx, int ty, int tz, int k)
     return c;
 }
 
-// two fused 3D sweeps per launch: tile (cells per lane, rows per wave), block
-// (threads, waves along d); the segment length is chosen at launch
-pskel_launch_cfg_t fused(int vd, int rw, int threads, int nwd)
+// fused 3D sweeps per launch (depth 2 or 3): tile (cells per lane, rows per
+// wave), block (threads, waves along d); the segment length is chosen at launch
+pskel_launch_cfg_t fused(int vd, int rw, int threads, int nwd, int depth = 2)
 {
-    pskel_launch_cfg_t c = tuned(vd, rw, 0, 2);
+    pskel_launch_cfg_t c = tuned(vd, rw, 0, depth);
     c.block_x = threads;
     c.block_y = nwd;
     return c;
@@ -128,6 +128,7 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
                 v.push_back(fused(2, 4, 1024, 4));
             }
         }
+        if (q.iterations >= 3 && !narrow) v.push_back(fused(4, 4, 512, 2, 3));  // three sweeps per launch (f32)
         v.push_back(plain(64, 4));
     } else {
         const int blocks[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 2}, {256, 1}, {32, 4}};
@@ -147,14 +148,14 @@ float fill3d(const pskel_tune_query_t& q, const pskel_launch_cfg_t& c)
     auto fit = [](double n, double t) { return n / (std::ceil(n / t) * t); };
     if (c.variant != PSKEL_VARIANT_TUNED)  // plain: block_x along d, block_y along w
         return (float)(fit(D, std::max(c.block_x, 1)) * fit(W, std::max(c.block_y, 1)));
-    if (c.time_tile >= 2) {  // fused pair: extended tiles overlapping by a lane vector and two rows
+    if (c.time_tile >= 2) {  // fused sweeps: extended tiles overlapping by a lane vector and 2(k-1) rows
         const int vd = c.tile_x > 0 ? c.tile_x : 4, rw = c.tile_y > 0 ? c.tile_y : 3;
         const int nw = c.block_x > 0 ? c.block_x / 64 : 12;
         const int nwd = c.block_y > 0 ? c.block_y % 16 : 2;
         if (nwd <= 0 || nw < nwd) return 1.f;
         const double td = 64.0 * nwd * vd, tw = (double)(nw / nwd) * rw;
         const double ntd = D <= td ? 1.0 : 1.0 + std::ceil((D - td) / (td - 2.0 * vd));
-        const double ntw = W <= tw ? 1.0 : 1.0 + std::ceil((W - tw) / (tw - 2.0));
+        const double ntw = W <= tw ? 1.0 : 1.0 + std::ceil((W - tw) / (tw - 2.0 * (c.time_tile - 1)));
         return (float)(D / (ntd * td) * W / (ntw * tw));
     }
     // one sweep per launch: tiles of 64 lanes x VD cells by 4 waves x RW rows
@@ -232,6 +233,12 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
                 bi = i;
             }
         }
+        // f32 with three sweeps per launch when the grid fills at least 70%
+        // of its 512-cell rows (it beat every pair from 406 deep to 512 on the
+        // grids measured; at 277 .. 342 deep it lost by up to 5%)
+        const double ntd3 = q.depth <= 512 ? 1.0 : 1.0 + std::ceil((q.depth - 512.0) / 504.0);
+        if (q.storage == PSKEL_DTYPE_F32 && q.iterations >= 3 && (double)q.depth / (512.0 * ntd3) >= 0.7)
+            return fused(4, 4, 512, 2, 3);
         if (q.iterations >= 2 && bf >= 0.2f) return pairs[bi];
         // one sweep per launch: a 64-lane row as wide as the depth needs, and
         // segments short enough for about 1024 blocks

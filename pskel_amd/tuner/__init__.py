@@ -33,7 +33,11 @@ from .. import inputs
 from .. import kernels as K
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data")
-DEFAULT_MODEL = os.path.join(DATA_DIR, "costmodel.bin")
+SHIPPED_MODEL = os.path.join(DATA_DIR, "costmodel.bin")
+# PSKEL_TUNER_MODEL names another model file ("none": no model), as it does
+# for the library's own pskel_tuner_choose
+_ENV_MODEL = os.environ.get("PSKEL_TUNER_MODEL")
+DEFAULT_MODEL = SHIPPED_MODEL if _ENV_MODEL is None else ("" if _ENV_MODEL in ("", "none") else _ENV_MODEL)
 
 STORAGE = {"u8": L.DTYPE_U8, "f32": L.DTYPE_F32, "bf16": L.DTYPE_BF16, "f16": L.DTYPE_F16}
 
@@ -391,13 +395,37 @@ def collect(queries: Sequence[Query], reps: int = 3, log=None) -> List[Dict]:
 
 
 def rows_to_json(rows: Sequence[Dict]) -> List[Dict]:
-    return [{"shape": list(r["q"].shape), "storage": r["q"].storage, "stencil": r["q"].stencil_name,
-             "iterations": r["q"].iterations, "ci": r["ci"], "ms": r["ms"]} for r in rows]
+    """Timing rows as JSON, each candidate named by its launch configuration
+    (cfg_key), so a saved table stays valid when the candidate list grows."""
+    out = []
+    cands: Dict[Query, List[L.LaunchCfg]] = {}
+    for r in rows:
+        q = r["q"]
+        if q not in cands:
+            cands[q] = candidates(q)
+        out.append({"shape": list(q.shape), "storage": q.storage, "stencil": q.stencil_name,
+                    "iterations": q.iterations, "cfg": list(cfg_key(cands[q][r["ci"]])), "ms": r["ms"]})
+    return out
 
 
-def rows_from_json(items: Sequence[Dict]) -> List[Dict]:
-    return [{"q": Query(tuple(x["shape"]), x["storage"], x["stencil"], x["iterations"]), "ci": x["ci"],
-             "ms": x["ms"]} for x in items]
+def rows_from_json(items: Sequence[Dict]) -> Tuple[List[Dict], List[Tuple[Query, L.LaunchCfg]]]:
+    """Rows of a saved table whose configuration is still a candidate, and the
+    (query, candidate) pairs the table lacks."""
+    rows, seen = [], {}
+    for x in items:
+        q = Query(tuple(x["shape"]), x["storage"], x["stencil"], x["iterations"])
+        if q not in seen:
+            seen[q] = ({cfg_key(c): i for i, c in enumerate(candidates(q))}, set())
+        index, have = seen[q]
+        k = tuple(x["cfg"])
+        if k in index:
+            rows.append({"q": q, "ci": index[k], "ms": x["ms"]})
+            have.add(k)
+    missing = []
+    for q, (index, have) in seen.items():
+        cands = candidates(q)
+        missing += [(q, cands[i]) for k, i in index.items() if k not in have]
+    return rows, missing
 
 
 def dataset(rows: Sequence[Dict]) -> Tuple[np.ndarray, np.ndarray]:

@@ -10,12 +10,15 @@ library's HIP kernels and ships their mean (packed into one model file), then
 measures M held-out queries exhaustively and reports, per query, the model's
 regret (time of its choice / best time - 1), the heuristic's and the GA's
 regret and measurement count, and the speed-up of the model's choice over the
-plain kernel.  --save-rows keeps both timing tables; --load-rows trains and
-reports from a saved pair instead of measuring again.
+plain kernel.  --save-rows keeps both timing tables (gzip when the name ends
+in .gz); --load-rows trains and reports from a saved pair instead of measuring
+again, timing only candidates added since.  The shipped model is trained from
+pskel_amd/data/tuner_rows.json.gz.
 """
 from __future__ import annotations
 
 import argparse
+import gzip
 import json
 import math
 import sys
@@ -24,7 +27,7 @@ import time
 import numpy as np
 
 from . import (CostModel, Measurer, candidates, cfg_dict, cfg_key, collect, dataset, exhaustive, genetic,
-               pack_ensemble, random_queries, regret, rows_from_json, rows_to_json, DEFAULT_MODEL)
+               pack_ensemble, random_queries, regret, rows_from_json, rows_to_json, SHIPPED_MODEL)
 
 
 def main(argv=None):
@@ -32,7 +35,7 @@ def main(argv=None):
     ap.add_argument("--train", type=int, default=160)
     ap.add_argument("--test", type=int, default=40)
     ap.add_argument("--epochs", type=int, default=600)
-    ap.add_argument("--out", default=DEFAULT_MODEL)
+    ap.add_argument("--out", default=SHIPPED_MODEL)
     ap.add_argument("--report", default="")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--ensemble", type=int, default=5, help="members averaged into the shipped model")
@@ -42,19 +45,38 @@ def main(argv=None):
 
     log = lambda m: print(m, flush=True)  # noqa: E731
     t0 = time.time()
+    def opener(path, mode):
+        return gzip.open(path, mode + "t") if path.endswith(".gz") else open(path, mode)
+
     if args.load_rows:
-        with open(args.load_rows) as f:
+        with opener(args.load_rows, "r") as f:
             saved = json.load(f)
-        rows, test_rows = rows_from_json(saved["train"]), rows_from_json(saved["test"])
+        rows, miss = rows_from_json(saved["train"])
+        test_rows, miss_t = rows_from_json(saved["test"])
+        # candidates added since the table was measured: time them now
+        for table, missing in ((rows, miss), (test_rows, miss_t)):
+            for q in dict.fromkeys(q for q, _ in missing):
+                m = Measurer(q)
+                try:
+                    keys = [cfg_key(c) for c in candidates(q)]
+                    for qq, c in missing:
+                        if qq == q:
+                            table.append({"q": q, "ci": keys.index(cfg_key(c)), "ms": m(c)})
+                finally:
+                    m.close()
+        log(f"loaded {len(rows)} + {len(test_rows)} timings, measured {len(miss) + len(miss_t)} new candidates")
         train_q = list(dict.fromkeys(r["q"] for r in rows))
         test_q = list(dict.fromkeys(r["q"] for r in test_rows))
+        if args.save_rows:
+            with opener(args.save_rows, "w") as f:
+                json.dump({"train": rows_to_json(rows), "test": rows_to_json(test_rows)}, f)
     else:
         train_q = random_queries(args.train, args.seed)
         rows = collect(train_q, log=log if args.train <= 400 else None)
         test_q = random_queries(args.test, args.seed + 1)
         test_rows = collect(test_q)
         if args.save_rows:
-            with open(args.save_rows, "w") as f:
+            with opener(args.save_rows, "w") as f:
                 json.dump({"train": rows_to_json(rows), "test": rows_to_json(test_rows)}, f)
     X, y = dataset(rows)
     log(f"training set: {len(y)} timings from {len(train_q)} queries ({time.time() - t0:.0f} s)")

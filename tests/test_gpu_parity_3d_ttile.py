@@ -1,6 +1,6 @@
-"""Two fused 3D sweeps per launch (time_tile=2, 7-point Jacobi) against the CPU
-oracle, bit-exact: the fused kernel evaluates both sweeps with the single
-sweep's expression, and with 16-bit storage rounds the first sweep to storage
+"""Two or three fused 3D sweeps per launch (time_tile=2 or 3, 7-point Jacobi)
+against the CPU oracle, bit-exact: the fused kernel evaluates every sweep with
+the single sweep's expression, and with 16-bit storage rounds the first sweep to storage
 (nearest, or stochastically with that sweep's own words) before the second
 reads it, so any split of N sweeps into launches gives the same words.
 Residuals of fused launches compare ||u^k - u^(k-1)|| against the oracle's to
@@ -137,13 +137,120 @@ def test_fused_rejects(gpu):
     with api.Grid(a.shape, "f32") as g:
         g.upload_pair(a)
         with pytest.raises(PSkelError):
-            g.run(K.jacobi3d(), 4, variant="tuned", time_tile=3)
+            g.run(K.jacobi3d(), 4, variant="tuned", time_tile=4)
         with pytest.raises(PSkelError):
             g.run(K.jacobi3d(), 4, variant="tuned", tile=(8, 3, 16), time_tile=2)
         with pytest.raises(PSkelError):  # not an instantiated shape
             g.run(K.jacobi3d(), 4, variant="tuned", block=_block(4, 2, 2), tile=(2, 4, 16), time_tile=2)
         with pytest.raises(PSkelError):  # threads not a whole count of waves along d
             g.run(K.jacobi3d(), 4, variant="tuned", block=(64 * 6, 4), tile=(2, 4, 16), time_tile=2)
+        # depth 3 exists for one shape (4 cells x 4 rows, 2 x 4 waves)
+        with pytest.raises(PSkelError):
+            g.run(K.jacobi3d(), 4, variant="tuned", time_tile=3, block=_block(2, 6, 1), tile=(4, 3, 0))
+    with api.Grid(a.shape, "bf16") as g:
+        g.upload_pair(a)
+        with pytest.raises(PSkelError):
+            g.run(K.jacobi3d(), 4, variant="tuned", time_tile=3)
+
+
+# three fused sweeps per launch (f32): the one instantiated shape
+T3 = (4, 4, 2, 4, 1)
+
+
+def _run3(a, iters, per, **kw):
+    vd, rw, nwd, nww, pf = T3
+    with api.Grid(a.shape, "f32") as g:
+        g.upload_pair(a)
+        st = g.run(K.jacobi3d(), iters, variant="tuned", block=_block(nwd, nww, pf), tile=(vd, rw, per),
+                   time_tile=3, **kw)
+        return g.download(), st
+
+
+@pytest.mark.parametrize("grid", GRIDS + [(40, 47, 515)], ids=["x".join(map(str, g)) for g in GRIDS + [(40, 47, 515)]])
+def test_fused_depth3_matches_oracle(gpu, grid):
+    """Three sweeps per launch; runs of 1 .. 7 sweeps end with a pair or a
+    single sweep.  Tiles overlap by 4 rows along w and one lane vector along d."""
+    a = _field(grid, sum(grid) + 3)
+    for per in (3, 0):
+        for iters in (1, 2, 3, 4, 5, 7):
+            got, st = _run3(a, iters, per)
+            want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (grid, per, iters)
+            assert st["launches"] == -(-iters // 3)
+
+
+def test_fused_depth3_default_shape(gpu):
+    """time_tile=3 with no tile or block: the depth-3 default shape, and the
+    depth-2 default for a closing pair."""
+    shape = (23, 61, 140)
+    a = _field(shape, 21)
+    for iters in (5, 6):
+        with api.Grid(shape, "f32") as g:
+            g.upload_pair(a)
+            g.run(K.jacobi3d(), iters, variant="tuned", time_tile=3)
+            got = g.download()
+        want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), iters
+
+
+def test_fused_depth3_tiles_segments_range(gpu):
+    """Many w tiles, h segments of odd length down to 5 planes (each reads 3
+    planes beyond it on both sides), and a declared range of 2."""
+    shape = (70, 130, 520)
+    a = _field(shape, 17)
+    for per, iters in ((5, 6), (7, 9), (32, 8), (100, 4), (0, 7)):
+        got, _ = _run3(a, iters, per)
+        want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (per, iters)
+    s = dataclasses.replace(K.jacobi3d(), radius=2)
+    b = _field((21, 37, 90), 12)
+    with api.Grid(b.shape, "f32") as g:
+        g.upload_pair(b)
+        g.run(s, 7, variant="tuned", time_tile=3)
+        got = g.download()
+    want, _ = oracle.run_3d(s, b, 7)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("every", [1, 2, 3, 4, 5])
+def test_fused_depth3_residuals(gpu, every):
+    """Launches stop at every residual step; the residual of a depth-3 launch
+    is its third sweep against its second."""
+    shape = (40, 36, 100)
+    a = _field(shape, 8)
+    iters = 20
+    got_a, st = _run3(a, iters, 16, residual_every=every)
+    want, want_res = oracle.run_3d(K.jacobi3d(), a, iters, residual_every=every)
+    assert np.array_equal(got_a.view(np.uint32), want.view(np.uint32))
+    got = np.array(st["residuals"])
+    assert got.shape == want_res.shape == (iters // every,)
+    np.testing.assert_allclose(got, want_res, rtol=1e-12, atol=0)
+
+
+def test_fused_depth3_split_runs_graph_full_size(gpu):
+    """run(4); run(5) and graph replays equal single sweeps; 512^3 x 6 in two
+    launches."""
+    shape = (30, 50, 200)
+    a = _field(shape, 19)
+    want9, _ = oracle.run_3d(K.jacobi3d(), a, 9)
+    with api.Grid(shape, "f32") as g:
+        g.upload_pair(a)
+        g.run(K.jacobi3d(), 4, variant="tuned", time_tile=3)
+        g.run(K.jacobi3d(), 5, variant="tuned", time_tile=3)
+        assert np.array_equal(g.download().view(np.uint32), want9.view(np.uint32))
+    want18, _ = oracle.run_3d(K.jacobi3d(), a, 18)
+    with api.Grid(shape, "f32") as g:
+        g.upload_pair(a)
+        for rep in range(2):
+            g.run(K.jacobi3d(), 9, variant="tuned", time_tile=3, use_graph=True)
+            api.synchronize()
+            w = want9 if rep == 0 else want18
+            assert np.array_equal(g.download().view(np.uint32), w.view(np.uint32)), rep
+    big = inputs.jacobi3d_field(512, seed=4)
+    want, _ = oracle.run_3d(K.jacobi3d(), big, 6)
+    got, st = _run3(big, 6, 0)
+    assert st["launches"] == 2
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
 def test_fused_depth_is_a_hint(gpu):

@@ -38,7 +38,7 @@ def test_3d_candidates_fit_lds():
 def test_3d_fused_candidates_per_storage():
     """Two fused sweeps per launch are offered for the 7-point Jacobi with at
     least two sweeps: the shapes built for each storage (four for 16-bit, all
-    also built for f32)."""
+    also built for f32); three for f32 with at least three."""
     fused = lambda q: [c for c in T.candidates(q) if c.time_tile == 2]  # noqa: E731
     f32 = fused(_q((64, 64, 64), "f32", "jacobi3d", 10))
     b16 = fused(_q((64, 64, 64), "bf16", "jacobi3d", 10))
@@ -47,20 +47,32 @@ def test_3d_fused_candidates_per_storage():
         {(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in f32}
     assert fused(_q((64, 64, 64), "f32", "jacobi3d", 1)) == []
     assert fused(_q((64, 64, 64), "f16", "jacobi3d", 1)) == []
-    for storage in ("f32", "bf16", "f16"):
+    for storage in ("bf16", "f16"):
         h = T.heuristic_choice(_q((512, 512, 512), storage, "jacobi3d", 100))
         assert (h.time_tile, h.tile_x, h.tile_y, h.block_x, h.block_y) == (2, 4, 3, 768, 2)
+    for storage in ("f32", "bf16", "f16"):
         assert T.heuristic_choice(_q((512, 512, 512), storage, "jacobi3d", 1)).time_tile == 1
+    # three sweeps per launch: f32 only, one shape, from three sweeps up
+    d3 = lambda q: [c for c in T.candidates(q) if c.time_tile == 3]  # noqa: E731
+    assert [(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in d3(_q((64, 64, 64), "f32", "jacobi3d", 3))] == \
+        [(4, 4, 512, 2)]
+    assert d3(_q((64, 64, 64), "f32", "jacobi3d", 2)) == [] and d3(_q((64, 64, 64), "bf16", "jacobi3d", 9)) == []
 
 
 def test_3d_heuristic_fits_the_depth():
-    """The heuristic takes the fused pair whose 512-, 256- or 128-cell rows
-    the grid fills best, from 20% fill of its tiles up; below, one sweep per
+    """The heuristic takes three fused f32 sweeps when 512-cell tiles are 70%
+    full, else the fused pair whose 512-, 256- or 128-cell rows the grid fills
+    best, from 20% fill of its tiles up; below, one sweep per
     launch with a 64-lane row as wide as the depth needs and segments short
     enough for about 1024 blocks."""
     h = lambda shape, st: T.heuristic_choice(_q(shape, st, "jacobi3d", 100))  # noqa: E731
     key = lambda c: (c.time_tile, c.tile_x, c.tile_y, c.block_x, c.block_y)  # noqa: E731
-    assert key(h((512, 512, 512), "f32")) == (2, 4, 3, 768, 2)
+    # f32: three sweeps per launch once 512-cell tiles are 70% full
+    assert key(h((512, 512, 512), "f32")) == (3, 4, 4, 512, 2)
+    assert key(h((479, 323, 406), "f32")) == (3, 4, 4, 512, 2)
+    assert key(h((349, 507, 323), "f32"))[0] == 2
+    assert key(h((410, 287, 517), "f32"))[0] == 2  # two 512-cell rows would be half empty
+    assert key(h((512, 512, 512), "bf16")) == (2, 4, 3, 768, 2)
     assert key(h((401, 258, 252), "f16")) == (2, 4, 3, 768, 1)
     assert key(h((334, 359, 199), "f32")) == (2, 4, 3, 768, 1)
     assert key(h((211, 243, 66), "f32")) == (2, 2, 3, 768, 1)
@@ -141,3 +153,21 @@ def test_packed_ensemble_is_the_members_mean():
             want = np.mean([oracle.mlp_predict(sizes, p.astype(np.float64), x) for p in ps], axis=0)
             got = oracle.mlp_predict(big, flat.astype(np.float64), x)
             np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
+
+
+def test_saved_timing_rows_round_trip():
+    """rows_to_json names each candidate by its launch configuration; loading
+    maps it back to the current candidate index and lists what is missing."""
+    qs = T.random_queries(8, 3)
+    rows = [{"q": q, "ci": i, "ms": 1.0 + i} for q in qs for i in range(len(T.candidates(q)))]
+    js = T.rows_to_json(rows)
+    back, missing = T.rows_from_json(js)
+    assert not missing and len(back) == len(rows)
+    assert all(a["ci"] == b["ci"] and a["q"] == b["q"] and a["ms"] == b["ms"] for a, b in zip(rows, back))
+    back, missing = T.rows_from_json(js[:-3])
+    assert len(back) == len(rows) - 3 and len(missing) == 3
+    assert {T.cfg_key(c) for _, c in missing} == {tuple(x["cfg"]) for x in js[-3:]}
+    # a configuration that is no longer a candidate is dropped
+    js[0] = dict(js[0], cfg=[99, 9, 9, 9, 9, 9, 9])
+    back, missing = T.rows_from_json(js)
+    assert len(back) == len(rows) - 1 and len(missing) == 1
