@@ -189,6 +189,7 @@ def measure(name, variant, steps, warmup, field=None, run_kw=None):
             kw = T.cfg_dict(tuner_pick(name))
         else:
             kw = dict(TUNED[name])
+    variant = kw.pop("variant", variant)  # the tuner may pick the plain kernel
     if w.get("residual_every"):
         kw["residual_every"] = w["residual_every"]
     f = field_of(name) if field is None else field
@@ -604,21 +605,27 @@ def cpu_baseline(name, seconds_target=12.0):
     }
 
 
-# rocprofv3 summary of the dominant kernel of each workload (kernel trace + PMC traffic)
+# rocprofv3 summary of the dominant kernel of each (workload, variant) (kernel
+# trace + PMC traffic), for the launch shape the tuner picks
 PROFILES = {
-    "jacobi3d": "k3d_march2_jacobi3d_512_f32.txt",
-    "jacobi3d_single": "k3d_march_jacobi3d_512_f32.txt",
-    "jacobi3d_bf16": "k3d_march2_jacobi3d_512_bf16_stochastic.txt",
-    "jacobi3d_bf16_single": "k3d_march_jacobi3d_512_bf16_stochastic.txt",
+    ("jacobi3d", "tuned"): "k3d_march2_depth3_jacobi3d_512_f32.txt",
+    ("jacobi3d", "plain"): "k3d_plain_jacobi3d_512_f32.txt",
+    ("jacobi3d_pair", "tuned"): "k3d_march2_jacobi3d_512_f32.txt",
+    ("jacobi3d_single", "tuned"): "k3d_march_jacobi3d_512_f32.txt",
+    ("jacobi3d_bf16", "tuned"): "k3d_march2_jacobi3d_512_bf16_stochastic.txt",
+    ("jacobi3d_bf16_single", "tuned"): "k3d_march_jacobi3d_512_bf16_stochastic.txt",
+    ("life", "tuned"): "k_life_bits_life_8192_u8.txt",
+    ("jacobi2d", "tuned"): "k2d_ttile_jacobi2d_2048_f32.txt",
+    ("diffusion9", "tuned"): "k2d_ttile_diffusion9_4096_f32.txt",
 }
 
 
-def traffic_from_profiles(name):
+def traffic_from_profiles(name, variant="tuned"):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
-    if name not in PROFILES:
+    if (name, variant) not in PROFILES:
         return None
-    p = os.path.join(ROOT, "profiles", PROFILES[name])
+    p = os.path.join(ROOT, "profiles", PROFILES[(name, variant)])
     if not os.path.exists(p):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read(), re.M)
@@ -721,6 +728,10 @@ def main(argv=None):
                                                        "avg_launch_ms", "sweeps_per_launch",
                                                        "effective_roofline_frac", "residuals_per_step", "launch")}
             extra[f"{nm}:{var}"]["hbm_roofline_frac"] = hbm_frac(r)
+            tr = traffic_from_profiles(nm, var)
+            if tr:  # HBM bytes per launch the committed profile measured
+                extra[f"{nm}:{var}"]["traffic_bytes_per_launch"] = tr
+                extra[f"{nm}:{var}"]["traffic_gbs"] = round(tr / (r["avg_launch_ms"] * 1e-3) / 1e9, 1)
             print(f"# {nm} {var}: {r['cell_updates_per_s']:.4g} CU/s, {hbm_frac(r)*100:.1f}% of HBM "
                   f"({r['effective_roofline_frac']*100:.0f}% effective)", file=sys.stderr, flush=True)
         rh = measure_host_tiled()
@@ -758,7 +769,9 @@ def main(argv=None):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profiles(args.workload),
+        "traffic": traffic_from_profiles(args.workload, args.variant),
+        "profile": (f"profiles/{PROFILES[(args.workload, args.variant)]}"
+                    if (args.workload, args.variant) in PROFILES else None),
         "kernel": ((f"k3d_march2 (depth {head['launch'].get('time_tile', 1)})"
                     if head["launch"].get("time_tile", 1) >= 2 else "k3d_march")
                    if args.workload.startswith("jacobi3d") and args.variant == "tuned" else None),

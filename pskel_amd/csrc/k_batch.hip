@@ -8,7 +8,8 @@
 // after its last item), each item running the body of the standalone kernels
 // with the grid's kernel family (Jacobi f32, 3x3 weighted f32, life u8) and
 // tile shape.  The sequence of launches is captured once in a hipGraph and
-// replayed.
+// replayed; each family's kernels run on a stream of their own, side by side
+// with the other families' (no grid belongs to two).
 //
 // Temporal depth.  Each grid fuses K sweeps per launch (K odd: 1 or 3 for the
 // f32 stencils, through the LDS temporal tile; 1, 3, 5 or 7 for life, through
@@ -178,6 +179,11 @@ struct pskel_batch {
     size_t sched_sweeps = 0;        // iteration count the schedule and item order are for
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // one stream per kernel family (default priority): no grid belongs to two
+    // families, so the families' launch sequences run side by side, forked
+    // from and joined back into the issuing stream by events
+    hipStream_t fam_stream[BF_COUNT] = {};
+    hipEvent_t fork = nullptr, join[BF_COUNT] = {};
     hipGraphExec_t exec = nullptr;
     size_t graph_sweeps = 0;
 };
@@ -230,23 +236,30 @@ pskel_status_t refresh_table(pskel_batch_t* b)
     return PSKEL_OK;
 }
 
+// Each family's launches in order on the family's stream, forked from `st`
+// and joined back into it (under capture: parallel branches of the graph).
 hipError_t issue_batch(pskel_batch_t* b, hipStream_t st)
 {
     const dim3 block(kWave, kBatchWaves);
-    for (int l = 0; l < b->nlaunch; ++l) {
-        for (int f = 0; f < BF_COUNT; ++f) {
+    hipError_t e = hipEventRecord(b->fork, st);
+    for (int f = 0; f < BF_COUNT && e == hipSuccess; ++f) {
+        if (b->fam_items[f] == 0) continue;
+        hipStream_t fs = b->fam_stream[f];
+        e = hipStreamWaitEvent(fs, b->fork, 0);
+        const int* items = b->d_items + b->fam_first[f];
+        for (int l = 0; l < b->nlaunch && e == hipSuccess; ++l) {
             const int n = b->launch_items[f][l];
             if (n == 0) continue;
-            const int* items = b->d_items + b->fam_first[f];
             const dim3 grid((unsigned)std::min(n, b->fam_blocks[f]));
-            if (f == BF_REG) hipLaunchKernelGGL(k_batch<BF_REG>, grid, block, 0, st, b->d_tab, items, n, l);
-            else if (f == BF_TTILE) hipLaunchKernelGGL(k_batch<BF_TTILE>, grid, block, 0, st, b->d_tab, items, n, l);
-            else hipLaunchKernelGGL(k_batch<BF_BITS>, grid, block, 0, st, b->d_tab, items, n, l);
-            const hipError_t e = hipGetLastError();
-            if (e != hipSuccess) return e;
+            if (f == BF_REG) hipLaunchKernelGGL(k_batch<BF_REG>, grid, block, 0, fs, b->d_tab, items, n, l);
+            else if (f == BF_TTILE) hipLaunchKernelGGL(k_batch<BF_TTILE>, grid, block, 0, fs, b->d_tab, items, n, l);
+            else hipLaunchKernelGGL(k_batch<BF_BITS>, grid, block, 0, fs, b->d_tab, items, n, l);
+            e = hipGetLastError();
         }
+        if (e == hipSuccess) e = hipEventRecord(b->join[f], fs);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, b->join[f], 0);
     }
-    return hipSuccess;
+    return e;
 }
 
 }  // namespace
@@ -330,6 +343,11 @@ pskel_status_t pskel_batch_create(pskel_grid_t* const* grids, const pskel_stenci
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (int f = 0; f < BF_COUNT && e == hipSuccess; ++f) {
+        e = hipStreamCreateWithFlags(&b->fam_stream[f], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[f], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = hipMalloc(&b->d_tab, b->host_tab.size() * sizeof(BatchEntry));
     if (e == hipSuccess) e = hipMalloc(&b->d_items, (size_t)b->nitems * sizeof(int));
     if (e != hipSuccess) {
@@ -351,6 +369,12 @@ pskel_status_t pskel_batch_destroy(pskel_batch_t* b)
     if (b->d_items) (void)hipFree(b->d_items);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (int f = 0; f < BF_COUNT; ++f) {
+        if (b->fam_stream[f]) (void)hipStreamSynchronize(b->fam_stream[f]);
+        if (b->fam_stream[f]) (void)hipStreamDestroy(b->fam_stream[f]);
+        if (b->join[f]) (void)hipEventDestroy(b->join[f]);
+    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return PSKEL_OK;

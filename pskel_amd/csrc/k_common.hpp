@@ -202,6 +202,33 @@ __device__ __forceinline__ void store_vec(T* __restrict__ p, const T (&in)[N])
     }
 }
 
+// Neighbour exchange across lanes: lane l receives lane l-1's value
+// (wave_shr:1) or lane l+1's (wave_shl:1); the lane with no source (lane 0,
+// respectively 63) keeps `edge`.  DPP controls of the row-shift family on
+// wave64 GFX9 parts (gfx950 included): a VALU modifier, no LDS round trip.
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppWaveShl1 = 0x130;
+
+__device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, uint32_t edge)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, kDppWaveShr1, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ uint32_t from_upper_lane(uint32_t v, uint32_t edge)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, kDppWaveShl1, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ float from_lower_lane(float v, float edge)
+{
+    return __uint_as_float(from_lower_lane(__float_as_uint(v), __float_as_uint(edge)));
+}
+
+__device__ __forceinline__ float from_upper_lane(float v, float edge)
+{
+    return __uint_as_float(from_upper_lane(__float_as_uint(v), __float_as_uint(edge)));
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic FP64 block reduction: wave butterfly, then waves in order.
 // Every thread of the block must call it.  Returns the block sum in thread 0.

@@ -147,10 +147,8 @@ __global__ __launch_bounds__(256) void k2d_cloud_tuned(Sweep2D s, CloudArgs a)
     float lf[RY + 2], rt[RY + 2];
 #pragma unroll
     for (int i = 0; i < RY + 2; ++i) {
-        const float up = __shfl_up(v[i].w, 1, kWave);
-        const float dn = __shfl_down(v[i].x, 1, kWave);
-        lf[i] = lane == 0 ? lfl[i] : up;
-        rt[i] = lane == kWave - 1 ? rtl[i] : dn;
+        lf[i] = from_lower_lane(v[i].w, lfl[i]);  // lane 0: the cell loaded for it
+        rt[i] = from_upper_lane(v[i].x, rtl[i]);  // lane 63: likewise
     }
     if (!col_live) return;
     const bool full = (w >= 1) && (w + 3 < Wd - 1);

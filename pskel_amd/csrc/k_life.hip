@@ -57,9 +57,10 @@ __global__ __launch_bounds__(256) void k_life_bits(Sweep2D s, int ntx, int ntile
 
 hipError_t launch_life_bits(const Sweep2D& s, int k, const pskel_launch_cfg_t& cfg, hipStream_t st)
 {
-    // rows per lane: cfg.tile_y rounded up to 32, 40, 48 or 64, and at least 2k + 16
-    int r = std::max(cfg.tile_y > 0 ? cfg.tile_y : 32, 2 * k + 16);
-    r = r <= 32 ? 32 : (r <= 40 ? 40 : (r <= 48 ? 48 : 64));
+    // rows per lane: cfg.tile_y rounded up to 24, 32, 40, 48 or 64, and at
+    // least 2k + 8 (8 core rows)
+    int r = std::max(cfg.tile_y > 0 ? cfg.tile_y : 32, 2 * k + 8);
+    r = r <= 24 ? 24 : (r <= 32 ? 32 : (r <= 40 ? 40 : (r <= 48 ? 48 : 64)));
     constexpr int CW = 62 * 32;
     const int ntx = std::max(1, (s.W - 1 + CW - 1) / CW);
 #define PSKEL_LIFE_BITS(KV, RV)                                                                     \
@@ -70,8 +71,9 @@ hipError_t launch_life_bits(const Sweep2D& s, int k, const pskel_launch_cfg_t& c
                            ntiles);                                                                 \
         return hipGetLastError();                                                                   \
     }
-#define PSKEL_LIFE_BITS_K(KV) \
-    PSKEL_LIFE_BITS(KV, 32) PSKEL_LIFE_BITS(KV, 40) PSKEL_LIFE_BITS(KV, 48) PSKEL_LIFE_BITS(KV, 64)
+#define PSKEL_LIFE_BITS_K(KV)                                                                        \
+    PSKEL_LIFE_BITS(KV, 24) PSKEL_LIFE_BITS(KV, 32) PSKEL_LIFE_BITS(KV, 40) PSKEL_LIFE_BITS(KV, 48) \
+    PSKEL_LIFE_BITS(KV, 64)
     PSKEL_LIFE_BITS_K(2)
     PSKEL_LIFE_BITS_K(3)
     PSKEL_LIFE_BITS_K(4)

@@ -72,10 +72,11 @@ inline March2Shape march2_shape(const Sweep3D& s, const pskel_launch_cfg_t& cfg,
     m.depth = depth;
     // default at depth 2: 18-row tiles of 2 x 6 waves (16 rows advanced per
     // tile, so a 512-wide grid takes 32 tiles and 8 segments fill the 256 CUs
-    // of an MI355X); at depth 3: 16-row tiles of 2 x 4 waves of 4 rows
+    // of an MI355X); at depth 3: 12-row tiles of 2 x 6 waves of 2 rows (8 rows
+    // advanced: 64 tiles and 4 segments)
     m.vd = cfg.tile_x > 0 ? cfg.tile_x : 4;
-    m.rw = cfg.tile_y > 0 ? cfg.tile_y : (depth == 3 ? 4 : 3);
-    const int nw = cfg.block_x > 0 ? cfg.block_x / kWave : (depth == 3 ? 8 : 12);
+    m.rw = cfg.tile_y > 0 ? cfg.tile_y : (depth == 3 ? 2 : 3);
+    const int nw = cfg.block_x > 0 ? cfg.block_x / kWave : 12;
     m.nwd = cfg.block_y > 0 ? cfg.block_y % 16 : 2;
     m.pf = cfg.block_y > 0 ? 1 + cfg.block_y / 16 : 1;
     m.nww = m.nwd > 0 ? nw / m.nwd : 0;
@@ -88,8 +89,9 @@ inline March2Shape march2_shape(const Sweep3D& s, const pskel_launch_cfg_t& cfg,
 
 // the instantiated shapes (vd, rw, nwd, nww, pf): at depth 2 ten for f32 and
 // four for 16-bit storage (rows of 512, 256 and 128 cells along d); at depth 3
-// one for f32 (16-row tiles of 512 cells: 2 x 4 waves of 4 rows x 64 lanes x 4
-// cells; the 12-wave shapes spill at depth 3)
+// two for f32, with tiles of 512 cells: 16 rows of 2 x 4 waves of 4 rows, and
+// 12 rows of 2 x 6 waves of 2 rows (the 12-wave shape of 3 rows spills at
+// depth 3); each also at depth 2 for the pair that closes a run
 inline bool march2_supported(const March2Shape& m, int storage)
 {
     auto key = [](int vd, int rw, int nwd, int nww, int pf) {
@@ -97,7 +99,7 @@ inline bool march2_supported(const March2Shape& m, int storage)
     };
     const int k = key(m.vd, m.rw, m.nwd, m.nww, m.pf);
     if (m.depth == 3)  // 8 waves: three register rings of 4 rows x 4 cells fit 2 waves per SIMD
-        return storage == PSKEL_DTYPE_F32 && k == key(4, 4, 2, 4, 1);
+        return storage == PSKEL_DTYPE_F32 && (k == key(4, 4, 2, 4, 1) || k == key(4, 2, 2, 6, 1));
     if (m.depth != 2) return false;
     if (storage != PSKEL_DTYPE_F32)
         return k == key(4, 3, 2, 6, 1) || k == key(4, 2, 2, 8, 1) || k == key(4, 3, 1, 12, 1) ||
@@ -105,7 +107,7 @@ inline bool march2_supported(const March2Shape& m, int storage)
     return k == key(2, 4, 4, 4, 1) || k == key(2, 4, 4, 2, 1) || k == key(4, 2, 2, 8, 1) ||
            k == key(4, 2, 2, 4, 1) || k == key(2, 4, 4, 4, 2) || k == key(2, 4, 2, 4, 1) ||
            k == key(4, 4, 2, 4, 1) || k == key(4, 3, 2, 6, 1) || k == key(4, 3, 1, 12, 1) ||
-           k == key(2, 3, 1, 12, 1);
+           k == key(2, 3, 1, 12, 1) || k == key(4, 2, 2, 6, 1);
 }
 
 hipError_t march3d_t2_f32(const Sweep3D& s, const Points& p, const March2Shape& m, hipStream_t st);
@@ -258,25 +260,6 @@ struct LaneWords {
     // the mask of cell j within its word
     static constexpr uint32_t cell_mask(int j) { return CPW == 1 ? ~0u : 0xffffu << (16 * (j % CPW)); }
 };
-
-// Neighbour exchange along d across lanes: lane l receives lane l-1's value
-// (wave_shr:1) or lane l+1's (wave_shl:1); the lane with no source (lane 0,
-// respectively 63) keeps `edge`, the neighbouring wave's cell from LDS.  DPP
-// controls of the row-shift family on wave64 GFX9 parts (gfx950 included).
-constexpr int kDppWaveShr1 = 0x138;
-constexpr int kDppWaveShl1 = 0x130;
-
-__device__ __forceinline__ float from_lower_lane(float v, float edge)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), kDppWaveShr1, 0xf,
-                                                      0xf, false));
-}
-
-__device__ __forceinline__ float from_upper_lane(float v, float edge)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), kDppWaveShl1, 0xf,
-                                                      0xf, false));
-}
 
 template <int ST, int VD, int RW, int NWD, int NWW, int PF, int DEPTH, bool RES, bool SR>
 __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float coeff, int hseg, int ntd, int ntw)
@@ -707,7 +690,9 @@ hipError_t march2_launch(const Sweep3D& s, const Points& p, const March2Shape& m
     X(4, 3, 2, 6, 1, 2)            \
     X(4, 3, 1, 12, 1, 2)           \
     X(2, 3, 1, 12, 1, 2)           \
-    X(4, 4, 2, 4, 1, 3)
+    X(4, 4, 2, 4, 1, 3)            \
+    X(4, 2, 2, 6, 1, 2)            \
+    X(4, 2, 2, 6, 1, 3)
 #define PSKEL_MARCH2_SHAPES_16(X) \
     X(4, 3, 2, 6, 1, 2)           \
     X(4, 2, 2, 8, 1, 2)           \

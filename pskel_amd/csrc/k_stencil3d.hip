@@ -58,20 +58,33 @@ __global__ __launch_bounds__(1024) void k3d_plain(Sweep3D s, Points p)
         } else {
             const int np = NP > 0 ? NP : p.np;
             float acc = 0.f;
+            if (!edge) {
+                // interior: every point in the grid, at the cell's index plus the
+                // point's element offset (uniform: computed on the scalar unit)
 #pragma unroll
-            for (int i = 0; i < (NP > 0 ? NP : kMaxPoints); ++i) {
-                if (NP == 0 && i >= np) break;
-                const int hh = h + p.dh[i], ww = w + p.dw[i], dd = d + p.dd[i];
-                float v;
-                if (!edge || ((unsigned)hh < (unsigned)H && (unsigned)ww < (unsigned)W &&
-                              (unsigned)dd < (unsigned)D)) {
-                    const T x = in[((size_t)hh * W + ww) * pitch + dd];
-                    v = DEV ? Tr::dec_raw(x) : Tr::dec(x, off);
-                } else {
-                    v = halo;
+                for (int i = 0; i < (NP > 0 ? NP : kMaxPoints); ++i) {
+                    if (NP == 0 && i >= np) break;
+                    const long long po = ((long long)p.dh[i] * W + p.dw[i]) * (long long)pitch + p.dd[i];
+                    const T x = in[(long long)idx + po];
+                    const float v = DEV ? Tr::dec_raw(x) : Tr::dec(x, off);
+                    if constexpr (KIND == PSKEL_KIND_SCALED_SUM) acc = (i == 0) ? v : acc + v;
+                    else acc = (i == 0) ? p.wt[i] * v : acc + p.wt[i] * v;
                 }
-                if constexpr (KIND == PSKEL_KIND_SCALED_SUM) acc = (i == 0) ? v : acc + v;
-                else acc = (i == 0) ? p.wt[i] * v : acc + p.wt[i] * v;
+            } else {
+#pragma unroll
+                for (int i = 0; i < (NP > 0 ? NP : kMaxPoints); ++i) {
+                    if (NP == 0 && i >= np) break;
+                    const int hh = h + p.dh[i], ww = w + p.dw[i], dd = d + p.dd[i];
+                    float v;
+                    if ((unsigned)hh < (unsigned)H && (unsigned)ww < (unsigned)W && (unsigned)dd < (unsigned)D) {
+                        const T x = in[((size_t)hh * W + ww) * pitch + dd];
+                        v = DEV ? Tr::dec_raw(x) : Tr::dec(x, off);
+                    } else {
+                        v = halo;
+                    }
+                    if constexpr (KIND == PSKEL_KIND_SCALED_SUM) acc = (i == 0) ? v : acc + v;
+                    else acc = (i == 0) ? p.wt[i] * v : acc + p.wt[i] * v;
+                }
             }
             if constexpr (DEV) acc = __builtin_fmaf(p.coeff, acc, s.c0);
             else if constexpr (KIND == PSKEL_KIND_SCALED_SUM) acc = p.coeff * acc;
@@ -193,7 +206,7 @@ bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg, int storage, int depth)
     Sweep3D s{};
     s.W = s.H = s.D = 1;
     const March2Shape m = march2_shape(s, cfg, depth);
-    const int threads = cfg.block_x > 0 ? cfg.block_x : (depth == 3 ? 8 : 12) * kWave;
+    const int threads = cfg.block_x > 0 ? cfg.block_x : 12 * kWave;
     return m.nwd > 0 && m.nww > 0 && m.nwd * m.nww * kWave == threads && march2_supported(m, storage);
 }
 

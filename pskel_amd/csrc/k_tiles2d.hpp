@@ -82,10 +82,8 @@ __device__ __forceinline__ void tile_f32(const Sweep2D& s, const PT& p, int tx, 
     float lf[RY + 2], rt[RY + 2];
 #pragma unroll
     for (int i = 0; i < RY + 2; ++i) {
-        const float up = __shfl_up(v[i].w, 1, kWave);
-        const float dn = __shfl_down(v[i].x, 1, kWave);
-        lf[i] = lane == 0 ? lfl[i] : up;
-        rt[i] = lane == kWave - 1 ? rtl[i] : dn;
+        lf[i] = from_lower_lane(v[i].w, lfl[i]);  // lane 0: the cell loaded for it
+        rt[i] = from_upper_lane(v[i].x, rtl[i]);  // lane 63: likewise
     }
     if (!col_live) return;
     const bool full = (w >= r) && (w + 3 < Wd - r);
@@ -188,10 +186,8 @@ __device__ __forceinline__ void tile_life(const Sweep2D& s, int tx, int ty)
     }
 #pragma unroll
     for (int i = 0; i < RY + 2; ++i) {
-        const uint32_t up = __shfl_up((uint32_t)(g[i].hi >> 56), 1, kWave);
-        const uint32_t dn = __shfl_down((uint32_t)(g[i].lo & 0xffu), 1, kWave);
-        g[i].l = lane == 0 ? el[i] : up;
-        g[i].r = lane == kWave - 1 ? er[i] : dn;
+        g[i].l = from_lower_lane((uint32_t)(g[i].hi >> 56), el[i]);
+        g[i].r = from_upper_lane((uint32_t)(g[i].lo & 0xffu), er[i]);
     }
     if (!col_live) return;
 
@@ -471,8 +467,8 @@ __device__ __forceinline__ void life_bits_tile(const Sweep2D& s, int ntx, int t,
 
     // horizontal sums of a row: H = left + centre + right, M = left + right
     auto hm = [&](uint32_t x, uint32_t& h0, uint32_t& h1, uint32_t& m0, uint32_t& m1) {
-        const uint32_t up = __shfl_up(x, 1, kWave);    // lane-1's word (lane 0: its own, a halo)
-        const uint32_t dn = __shfl_down(x, 1, kWave);  // lane+1's word (lane 63: its own, a halo)
+        const uint32_t up = from_lower_lane(x, x);  // lane-1's word (lane 0: its own, a halo)
+        const uint32_t dn = from_upper_lane(x, x);  // lane+1's word (lane 63: its own, a halo)
         const uint32_t l = (x << 1) | (up >> 31), r = (x >> 1) | (dn << 31);
         h0 = l ^ x ^ r;
         h1 = (l & x) | (r & (l ^ x));

@@ -102,7 +102,7 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
         for (auto& t : ry_nwy) v.push_back(tuned(0, t[0], t[1], 1));
         if (pat == PAT_LIFE8) {  // bit-packed tiles: rows per lane x generations per launch
             for (int k = 2; k <= 8; ++k)
-                for (int rows : {32, 40, 48}) v.push_back(tuned(0, rows, 0, k));
+                for (int rows : {24, 32, 40, 48}) v.push_back(tuned(0, rows, 0, k));
         } else if (pat != PAT_CLOUD) {  // field stencils run one sweep per launch
             for (int k = 2; k <= 4; ++k)
                 for (int cy : {8, 16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
@@ -126,9 +126,13 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
             if (!narrow) {
                 v.push_back(fused(4, 2, 512, 2));
                 v.push_back(fused(2, 4, 1024, 4));
+                v.push_back(fused(4, 2, 768, 2));  // 12-row tiles of 2 x 6 waves of 2 rows
             }
         }
-        if (q.iterations >= 3 && !narrow) v.push_back(fused(4, 4, 512, 2, 3));  // three sweeps per launch (f32)
+        if (q.iterations >= 3 && !narrow) {  // three sweeps per launch (f32)
+            v.push_back(fused(4, 4, 512, 2, 3));
+            v.push_back(fused(4, 2, 768, 2, 3));
+        }
         v.push_back(plain(64, 4));
     } else {
         const int blocks[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 2}, {256, 1}, {32, 4}};
@@ -209,7 +213,7 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
     if (q.ndim == 2 && pat == PAT_LIFE8) {
         const double cells = (double)q.width * q.height;
         // bit-packed tiles are 1984 columns wide: narrower grids leave lanes idle
-        if (q.iterations >= 8 && cells >= (1 << 20) && q.width >= 2048) return tuned(0, 32, 0, 8);
+        if (q.iterations >= 8 && cells >= (1 << 20) && q.width >= 2048) return tuned(0, 24, 0, 8);
         // small grids: one cell per lane keeps enough blocks in flight
         if (cells < (1 << 18)) return plain(64, 4);
         return tuned(0, 1, 4, 1);
@@ -238,7 +242,7 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
         // grids measured; at 277 .. 342 deep it lost by up to 5%)
         const double ntd3 = q.depth <= 512 ? 1.0 : 1.0 + std::ceil((q.depth - 512.0) / 504.0);
         if (q.storage == PSKEL_DTYPE_F32 && q.iterations >= 3 && (double)q.depth / (512.0 * ntd3) >= 0.7)
-            return fused(4, 4, 512, 2, 3);
+            return fused(4, 2, 768, 2, 3);
         if (q.iterations >= 2 && bf >= 0.2f) return pairs[bi];
         // one sweep per launch: a 64-lane row as wide as the depth needs, and
         // segments short enough for about 1024 blocks

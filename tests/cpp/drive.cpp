@@ -195,6 +195,7 @@ static int run2d(Mask mask, Args args, const std::string& method, size_t W, size
     if (!writef(files[2], out.hostArray, W * H)) return 4;
     in.hostFree();
     out.hostFree();
+    mask.hostFree();  // handles are shallow: the program frees what it allocated
     return 0;
 }
 
@@ -231,6 +232,7 @@ static int run3d(Mask mask, Args args, const std::string& method, size_t W, size
     if (std::strcmp(files[2], "-") != 0 && !writef(files[2], out.hostArray, n)) return 4;
     in.hostFree();
     out.hostFree();
+    mask.hostFree();
     return 0;
 }
 
@@ -271,6 +273,8 @@ static int runMap(M& m, A& in, A& out, const std::string& method, size_t iters, 
 #endif
     else return 2;
     if (!writef(files[2], out.hostArray, n)) return 4;
+    in.hostFree();
+    out.hostFree();
     return 0;
 }
 

@@ -25,6 +25,7 @@ int main()
                     t.coreDepthOffset, t.coreDepth);
         in.hostFree();
         out.hostFree();
+        mask.hostFree();
     }
     return 0;
 }

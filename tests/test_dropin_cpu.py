@@ -81,3 +81,31 @@ def test_dropin_map_cpu(kernel, shape, method):
         for _ in range(iters):
             want = (np.float32(0.5) * want * want + np.float32(0.125)).astype(np.float32)
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (kernel, method, iters)
+
+
+def test_dropin_cpu_under_sanitizers():
+    """The headers' CPU methods and StencilTiling::tile built with
+    AddressSanitizer + UndefinedBehaviorSanitizer (g++, host code): every
+    kernel family, both CPU methods, field arguments and Map, on small
+    ragged grids, with results equal to the oracle's."""
+    import os
+    import subprocess
+
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    os.environ.update({k: env[k] for k in ("ASAN_OPTIONS", "UBSAN_OPTIONS")})
+    for kernel, stencil, field in CASES:
+        a = field()
+        for method in ("cpu", "seq"):
+            rc, got, err = dropin.run("drive_cpu_asan", kernel, method, a, 3)
+            assert rc == 0 and "runtime error" not in err, err[-3000:]
+            want = oracle.run_3d(stencil(), a, 3)[0] if a.ndim == 3 else oracle.run_2d(stencil(), a, 3)
+            assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (kernel, method)
+    t, wx, wy = inputs.cloudsim_fields(13, 21, seed=9)
+    rc, got, err = dropin.run("drive_cpu_asan", "usercloud", "cpu", t, 2, wind=(wx, wy))
+    assert rc == 0 and "runtime error" not in err, err[-3000:]
+    rc, got, err = dropin.run("drive_cpu_asan", "map3d", "cpu", inputs.uniform(3, 0, 99).reshape(3, 3, 11), 2)
+    assert rc == 0 and "runtime error" not in err, err[-3000:]
+    p = subprocess.run([dropin.driver("tile_geometry_asan")], input="100 50 30 1 3 2 0 0 10 10 10\n7 5 3 2 2 0 4 1 9 9 9\n",
+                       capture_output=True, text=True, env=env, timeout=120)
+    assert p.returncode == 0 and "runtime error" not in p.stderr, p.stderr[-3000:]

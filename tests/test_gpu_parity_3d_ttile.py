@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 # (cells per lane, rows per wave, waves along d, waves along w, planes prefetched):
 # the instantiated fused-tile shapes
 SHAPES_T2 = [(2, 4, 4, 4, 1), (2, 4, 4, 2, 1), (4, 2, 2, 8, 1), (4, 2, 2, 4, 1), (2, 4, 4, 4, 2), (2, 4, 2, 4, 1),
-             (4, 4, 2, 4, 1), (4, 3, 2, 6, 1), (4, 3, 1, 12, 1), (2, 3, 1, 12, 1)]
+             (4, 4, 2, 4, 1), (4, 3, 2, 6, 1), (4, 3, 1, 12, 1), (2, 3, 1, 12, 1), (4, 2, 2, 6, 1)]
 
 # grids: tiny, one tile, ragged, many w tiles, several d tiles (D > 64 * 8)
 GRIDS = [(1, 1, 1), (2, 3, 4), (3, 3, 3), (5, 9, 70), (17, 9, 70), (33, 40, 65), (20, 70, 300), (9, 23, 1100)]
@@ -144,7 +144,7 @@ def test_fused_rejects(gpu):
             g.run(K.jacobi3d(), 4, variant="tuned", block=_block(4, 2, 2), tile=(2, 4, 16), time_tile=2)
         with pytest.raises(PSkelError):  # threads not a whole count of waves along d
             g.run(K.jacobi3d(), 4, variant="tuned", block=(64 * 6, 4), tile=(2, 4, 16), time_tile=2)
-        # depth 3 exists for one shape (4 cells x 4 rows, 2 x 4 waves)
+        # depth 3 exists for two shapes (4 cells x 4 rows, 2 x 4 waves; 4 cells x 2 rows, 2 x 6 waves)
         with pytest.raises(PSkelError):
             g.run(K.jacobi3d(), 4, variant="tuned", time_tile=3, block=_block(2, 6, 1), tile=(4, 3, 0))
     with api.Grid(a.shape, "bf16") as g:
@@ -153,12 +153,13 @@ def test_fused_rejects(gpu):
             g.run(K.jacobi3d(), 4, variant="tuned", time_tile=3)
 
 
-# three fused sweeps per launch (f32): the one instantiated shape
-T3 = (4, 4, 2, 4, 1)
+# three fused sweeps per launch (f32): the instantiated shapes
+SHAPES_T3 = [(4, 4, 2, 4, 1), (4, 2, 2, 6, 1)]
+T3 = SHAPES_T3[0]
 
 
-def _run3(a, iters, per, **kw):
-    vd, rw, nwd, nww, pf = T3
+def _run3(a, iters, per, tshape=T3, **kw):
+    vd, rw, nwd, nww, pf = tshape
     with api.Grid(a.shape, "f32") as g:
         g.upload_pair(a)
         st = g.run(K.jacobi3d(), iters, variant="tuned", block=_block(nwd, nww, pf), tile=(vd, rw, per),
@@ -166,14 +167,15 @@ def _run3(a, iters, per, **kw):
         return g.download(), st
 
 
+@pytest.mark.parametrize("tshape", SHAPES_T3, ids=["vd{}_rw{}_d{}_w{}_pf{}".format(*t) for t in SHAPES_T3])
 @pytest.mark.parametrize("grid", GRIDS + [(40, 47, 515)], ids=["x".join(map(str, g)) for g in GRIDS + [(40, 47, 515)]])
-def test_fused_depth3_matches_oracle(gpu, grid):
+def test_fused_depth3_matches_oracle(gpu, tshape, grid):
     """Three sweeps per launch; runs of 1 .. 7 sweeps end with a pair or a
     single sweep.  Tiles overlap by 4 rows along w and one lane vector along d."""
     a = _field(grid, sum(grid) + 3)
     for per in (3, 0):
         for iters in (1, 2, 3, 4, 5, 7):
-            got, st = _run3(a, iters, per)
+            got, st = _run3(a, iters, per, tshape)
             want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (grid, per, iters)
             assert st["launches"] == -(-iters // 3)
@@ -198,10 +200,11 @@ def test_fused_depth3_tiles_segments_range(gpu):
     planes beyond it on both sides), and a declared range of 2."""
     shape = (70, 130, 520)
     a = _field(shape, 17)
-    for per, iters in ((5, 6), (7, 9), (32, 8), (100, 4), (0, 7)):
-        got, _ = _run3(a, iters, per)
-        want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
-        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (per, iters)
+    for tshape in SHAPES_T3:
+        for per, iters in ((5, 6), (7, 9), (32, 8), (100, 4), (0, 7)):
+            got, _ = _run3(a, iters, per, tshape)
+            want, _ = oracle.run_3d(K.jacobi3d(), a, iters)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (tshape, per, iters)
     s = dataclasses.replace(K.jacobi3d(), radius=2)
     b = _field((21, 37, 90), 12)
     with api.Grid(b.shape, "f32") as g:
@@ -219,7 +222,7 @@ def test_fused_depth3_residuals(gpu, every):
     shape = (40, 36, 100)
     a = _field(shape, 8)
     iters = 20
-    got_a, st = _run3(a, iters, 16, residual_every=every)
+    got_a, st = _run3(a, iters, 16, SHAPES_T3[every % 2], residual_every=every)
     want, want_res = oracle.run_3d(K.jacobi3d(), a, iters, residual_every=every)
     assert np.array_equal(got_a.view(np.uint32), want.view(np.uint32))
     got = np.array(st["residuals"])
@@ -248,9 +251,10 @@ def test_fused_depth3_split_runs_graph_full_size(gpu):
             assert np.array_equal(g.download().view(np.uint32), w.view(np.uint32)), rep
     big = inputs.jacobi3d_field(512, seed=4)
     want, _ = oracle.run_3d(K.jacobi3d(), big, 6)
-    got, st = _run3(big, 6, 0)
-    assert st["launches"] == 2
-    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    for tshape in SHAPES_T3:
+        got, st = _run3(big, 6, 0, tshape)
+        assert st["launches"] == 2
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), tshape
 
 
 def test_fused_depth_is_a_hint(gpu):

@@ -41,7 +41,8 @@ def test_life_tuned_shapes(gpu, ry, nwy):
 @pytest.mark.parametrize("k", [2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("cy", [16, 32, 48, 64])
 def test_life_temporal(gpu, k, cy):
-    """k generations per launch on bit-packed tiles of cy (rounded) rows per lane."""
+    """k generations per launch on bit-packed tiles of cy rows per lane, rounded
+    up to 24, 32, 40, 48 or 64 and to at least 2k + 8."""
     b = inputs.life_field(500, seed=4)
     for iters in (k, 2 * k + 1, 11):
         got = _run(b, iters, variant="tuned", time_tile=k, tile=(0, cy, 0))
@@ -71,8 +72,9 @@ def test_life_temporal_ragged(gpu, shape, k):
     b = (inputs.uniform(31, 0, shape[0] * shape[1]).reshape(shape) < 0.4).astype(np.uint8)
     o = (inputs.uniform(32, 0, shape[0] * shape[1]).reshape(shape) < 0.5).astype(np.uint8)
     for iters in (k, k + 3):
-        got = _run(b, iters, out=o, variant="tuned", time_tile=k)
-        assert np.array_equal(got, oracle.run_2d(K.life(), b, iters, out=o)), (shape, k, iters)
+        for cy in (0, 24):  # default 32-row tiles, and 24-row tiles (8 core rows at k = 8)
+            got = _run(b, iters, out=o, variant="tuned", time_tile=k, tile=(0, cy, 0))
+            assert np.array_equal(got, oracle.run_2d(K.life(), b, iters, out=o)), (shape, k, iters, cy)
 
 
 def test_life_rejects_non_binary_cells(gpu):

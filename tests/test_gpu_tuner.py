@@ -53,7 +53,10 @@ def test_genetic_search_finds_a_good_candidate(gpu):
 @pytest.mark.skipif(not os.path.exists(T.DEFAULT_MODEL), reason="no shipped cost model")
 def test_shipped_model_regret_on_the_workloads(gpu):
     """The shipped model on the benchmark workloads' shapes (not its training
-    set): its choice is within 10% of the exhaustive best on every query."""
+    set): its choice is within 10% of the exhaustive best on every query.
+    A query whose table shows more is timed again, the pick against the
+    table's best, both with more repetitions, before it is judged: the small
+    grids are launch-bound, where one median of three can be off by 10%."""
     m = T.CostModel.load()
     try:
         qs = [T.Query((2048, 2048), "f32", "jacobi2d", 100), T.Query((8194, 8194), "u8", "life", 500),
@@ -62,7 +65,24 @@ def test_shipped_model_regret_on_the_workloads(gpu):
               T.Query((96, 200, 400), "bf16", "jacobi3d", 100)]
         rows = T.collect(qs, reps=3)
         for x in T.regret(m, rows):
-            assert x["regret"] <= 0.10, (x["q"], x["regret"])
+            reg = x["regret"]
+            if reg > 0.10:
+                q = x["q"]
+                t = {r["ci"]: r["ms"] for r in rows if r["q"] == q}
+                cands = T.candidates(q)
+                best = cands[min(t, key=t.get)]
+                pick = m.choose(q)
+                again = T.Measurer(q, reps=9)
+                try:  # best, pick, best again (the measurer caches by configuration)
+                    tb = again(best)
+                    again.cache.clear()
+                    tp = again(pick)
+                    again.cache.clear()
+                    tb = min(tb, again(best))
+                finally:
+                    again.close()
+                reg = tp / tb - 1.0
+            assert reg <= 0.10, (x["q"], x["regret"], reg)
     finally:
         m.close()
 

@@ -42,7 +42,7 @@ def test_3d_fused_candidates_per_storage():
     fused = lambda q: [c for c in T.candidates(q) if c.time_tile == 2]  # noqa: E731
     f32 = fused(_q((64, 64, 64), "f32", "jacobi3d", 10))
     b16 = fused(_q((64, 64, 64), "bf16", "jacobi3d", 10))
-    assert len(f32) == 6 and len(b16) == 4
+    assert len(f32) == 7 and len(b16) == 4
     assert {(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in b16} <= \
         {(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in f32}
     assert fused(_q((64, 64, 64), "f32", "jacobi3d", 1)) == []
@@ -55,7 +55,7 @@ def test_3d_fused_candidates_per_storage():
     # three sweeps per launch: f32 only, one shape, from three sweeps up
     d3 = lambda q: [c for c in T.candidates(q) if c.time_tile == 3]  # noqa: E731
     assert [(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in d3(_q((64, 64, 64), "f32", "jacobi3d", 3))] == \
-        [(4, 4, 512, 2)]
+        [(4, 4, 512, 2), (4, 2, 768, 2)]
     assert d3(_q((64, 64, 64), "f32", "jacobi3d", 2)) == [] and d3(_q((64, 64, 64), "bf16", "jacobi3d", 9)) == []
 
 
@@ -68,8 +68,8 @@ def test_3d_heuristic_fits_the_depth():
     h = lambda shape, st: T.heuristic_choice(_q(shape, st, "jacobi3d", 100))  # noqa: E731
     key = lambda c: (c.time_tile, c.tile_x, c.tile_y, c.block_x, c.block_y)  # noqa: E731
     # f32: three sweeps per launch once 512-cell tiles are 70% full
-    assert key(h((512, 512, 512), "f32")) == (3, 4, 4, 512, 2)
-    assert key(h((479, 323, 406), "f32")) == (3, 4, 4, 512, 2)
+    assert key(h((512, 512, 512), "f32")) == (3, 4, 2, 768, 2)
+    assert key(h((479, 323, 406), "f32")) == (3, 4, 2, 768, 2)
     assert key(h((349, 507, 323), "f32"))[0] == 2
     assert key(h((410, 287, 517), "f32"))[0] == 2  # two 512-cell rows would be half empty
     assert key(h((512, 512, 512), "bf16")) == (2, 4, 3, 768, 2)
