@@ -558,6 +558,11 @@ def accuracy_3d(sweeps=1000, every=50, n=512):
     return out
 
 
+# CPUs this process may run on, read at import: once OpenMP binds its threads
+# (OMP_PROC_BIND=close) the main thread's own mask shrinks to one place
+START_CPUS = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -601,7 +606,7 @@ def cpu_baseline(name, seconds_target=12.0):
         "cpu_model": cpu_model(),
         "omp": {"threads": threads, "proc_bind": os.environ.get("OMP_PROC_BIND"),
                 "places": os.environ.get("OMP_PLACES"),
-                "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None},
+                "affinity_cpus": START_CPUS},
     }
 
 

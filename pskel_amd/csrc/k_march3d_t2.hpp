@@ -175,6 +175,14 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
     }
 }
 
+// x when A, else y (references of one type; the other is unused)
+template <bool A, typename X, typename Y>
+__device__ __forceinline__ auto& either(X& x, Y& y)
+{
+    if constexpr (A) return x;
+    else return y;
+}
+
 template <int VD>
 __device__ __forceinline__ void lds_put(typename LVec<VD>::T* p, const float (&v)[VD])
 {
@@ -371,11 +379,17 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         }
 
         constexpr int NQ = PF + 3;  // ring slots: planes h-1 .. h+PF+1
+        constexpr bool F32 = ST == PSKEL_DTYPE_F32;
         Word in[NQ][RW][NV];        // input planes as loaded (storage words)
         Word hal[NQ][NV];           // input planes of the halo row
-        float fin[NQ][RW][VD];      // input planes h-1, h, h+1, decoded once each
         Word sw[NS][NQ][RW][NV];    // sweep l+1's planes, storage words
-        float fs[NS][NQ][RW][VD];   // sweep l+1's planes h-l-2, h-l-1, h-l, decoded once each
+        // the rings as floats: 16-bit storage decodes each plane once (input
+        // planes h-1, h, h+1; sweep l+1's planes h-l-2, h-l-1, h-l); f32 words
+        // are the floats, so the same registers serve (no copies to carry)
+        float fin_own[F32 ? 1 : NQ][RW][VD];
+        float fs_own[F32 ? 1 : NS][NQ][RW][VD];
+        auto& fin = either<F32>(in, fin_own);
+        auto& fs = either<F32>(sw, fs_own);
 
         auto load = [&](int p, Word (&v)[RW][NV], Word (&hv)[NV]) {
             const bool ok = p >= hlo && p < hhi;  // uniform across the block
@@ -385,10 +399,12 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
             buf_load<Word, NV>(rs, hl_off, hv);
         };
         auto decode = [&](const Word (&w)[RW][NV], float (&f)[RW][VD]) {
+            if constexpr (!F32) {
 #pragma unroll
-            for (int i = 0; i < RW; ++i)
+                for (int i = 0; i < RW; ++i)
 #pragma unroll
-                for (int j = 0; j < VD; ++j) f[i][j] = LW::raw(w[i], j);
+                    for (int j = 0; j < VD; ++j) f[i][j] = LW::raw(w[i], j);
+            }
         };
         // a plane's boundary rows and edge cells into LDS
         auto stage = [&](LV (*L)[NL], float (*E)[NWD][RW][2], const float (&v)[RW][VD]) {

@@ -344,8 +344,9 @@ def heuristic_choice(q: Query) -> L.LaunchCfg:
 # ---------------------------------------------------------------------------
 def random_queries(n: int, seed: int, kinds=("jacobi2d", "diffusion9", "life", "jacobi3d")) -> List[Query]:
     """Random shapes: 2D sides log-uniform in [64, 4096] for half the queries
-    of each 2D kind and in [1024, 8192] for the other half (at most 2^26
-    cells), where bit-packed life tiles and deep temporal tiles pay; 3D sides
+    of each 2D kind and in [1024, 8448] for the other half (at most 2^27
+    cells: 8192^2 boards with their ring included), where bit-packed life
+    tiles and deep temporal tiles pay; 3D sides
     log-uniform in [32, 512] for half the 3D queries and in [192, 640] for the
     other half, where the HBM-bound kernels part ways (at most 2^27 cells);
     iterations log-uniform in [10, 1000]."""
@@ -368,10 +369,10 @@ def random_queries(n: int, seed: int, kinds=("jacobi2d", "diffusion9", "life", "
         else:
             k2 = n2.get(kind, 0)
             n2[kind] = k2 + 1
-            lo, hi = (64, 4096) if k2 % 2 == 0 else (1024, 8192)
+            lo, hi = (64, 4096) if k2 % 2 == 0 else (1024, 8448)
             while True:
                 s = tuple(int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(2))
-                if np.prod(s) <= 2 ** 26:
+                if np.prod(s) <= 2 ** 27:
                     break
             out.append(Query(s, "u8" if kind == "life" else "f32", kind, it))
     return out
