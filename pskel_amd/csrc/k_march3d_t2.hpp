@@ -87,8 +87,8 @@ inline March2Shape march2_shape(const Sweep3D& s, const pskel_launch_cfg_t& cfg,
     return m;
 }
 
-// the instantiated shapes (vd, rw, nwd, nww, pf): at depth 2 ten for f32 and
-// four for 16-bit storage (rows of 512, 256 and 128 cells along d); at depth 3
+// the instantiated shapes (vd, rw, nwd, nww, pf): at depth 2 eleven for f32 and
+// five for 16-bit storage (rows of 512, 256 and 128 cells along d); at depth 3
 // two for f32, with tiles of 512 cells: 16 rows of 2 x 4 waves of 4 rows, and
 // 12 rows of 2 x 6 waves of 2 rows (the 12-wave shape of 3 rows spills at
 // depth 3); each also at depth 2 for the pair that closes a run
@@ -103,7 +103,7 @@ inline bool march2_supported(const March2Shape& m, int storage)
     if (m.depth != 2) return false;
     if (storage != PSKEL_DTYPE_F32)
         return k == key(4, 3, 2, 6, 1) || k == key(4, 2, 2, 8, 1) || k == key(4, 3, 1, 12, 1) ||
-               k == key(2, 3, 1, 12, 1);
+               k == key(2, 3, 1, 12, 1) || k == key(4, 2, 2, 6, 1);
     return k == key(2, 4, 4, 4, 1) || k == key(2, 4, 4, 2, 1) || k == key(4, 2, 2, 8, 1) ||
            k == key(4, 2, 2, 4, 1) || k == key(2, 4, 4, 4, 2) || k == key(2, 4, 2, 4, 1) ||
            k == key(4, 4, 2, 4, 1) || k == key(4, 3, 2, 6, 1) || k == key(4, 3, 1, 12, 1) ||
@@ -713,7 +713,8 @@ hipError_t march2_launch(const Sweep3D& s, const Points& p, const March2Shape& m
     X(4, 3, 2, 6, 1, 2)           \
     X(4, 2, 2, 8, 1, 2)           \
     X(4, 3, 1, 12, 1, 2)          \
-    X(2, 3, 1, 12, 1, 2)
+    X(2, 3, 1, 12, 1, 2)          \
+    X(4, 2, 2, 6, 1, 2)
 
 template <int ST>
 size_t march2_blocks(const Sweep3D& s, const March2Shape& m)

@@ -123,10 +123,10 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
             v.push_back(fused(4, 2, 1024, 2));
             v.push_back(fused(4, 3, 768, 1));  // 256-cell rows, 36-row tiles
             v.push_back(fused(2, 3, 768, 1));  // 128-cell rows
+            v.push_back(fused(4, 2, 768, 2));  // 12-row tiles of 2 x 6 waves of 2 rows
             if (!narrow) {
                 v.push_back(fused(4, 2, 512, 2));
                 v.push_back(fused(2, 4, 1024, 4));
-                v.push_back(fused(4, 2, 768, 2));  // 12-row tiles of 2 x 6 waves of 2 rows
             }
         }
         if (q.iterations >= 3 && !narrow) {  // three sweeps per launch (f32)

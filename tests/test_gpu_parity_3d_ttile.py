@@ -279,7 +279,7 @@ def test_fused_depth_is_a_hint(gpu):
     assert np.array_equal(got, want16)
 
 
-SHAPES_16 = [(4, 3, 2, 6, 1), (4, 2, 2, 8, 1), (4, 3, 1, 12, 1), (2, 3, 1, 12, 1)]
+SHAPES_16 = [(4, 3, 2, 6, 1), (4, 2, 2, 8, 1), (4, 3, 1, 12, 1), (2, 3, 1, 12, 1), (4, 2, 2, 6, 1)]
 
 
 def _run16(words, iters, store, tshape, per=0, offset=0.5, rounding="nearest", seed=0, **kw):

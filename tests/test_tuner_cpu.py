@@ -37,12 +37,12 @@ def test_3d_candidates_fit_lds():
 
 def test_3d_fused_candidates_per_storage():
     """Two fused sweeps per launch are offered for the 7-point Jacobi with at
-    least two sweeps: the shapes built for each storage (four for 16-bit, all
+    least two sweeps: the shapes built for each storage (five for 16-bit, all
     also built for f32); three for f32 with at least three."""
     fused = lambda q: [c for c in T.candidates(q) if c.time_tile == 2]  # noqa: E731
     f32 = fused(_q((64, 64, 64), "f32", "jacobi3d", 10))
     b16 = fused(_q((64, 64, 64), "bf16", "jacobi3d", 10))
-    assert len(f32) == 7 and len(b16) == 4
+    assert len(f32) == 7 and len(b16) == 5
     assert {(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in b16} <= \
         {(c.tile_x, c.tile_y, c.block_x, c.block_y) for c in f32}
     assert fused(_q((64, 64, 64), "f32", "jacobi3d", 1)) == []
