@@ -206,7 +206,7 @@ bool ttile3d_shape_ok(const pskel_launch_cfg_t& cfg, int storage, int depth)
     Sweep3D s{};
     s.W = s.H = s.D = 1;
     const March2Shape m = march2_shape(s, cfg, depth);
-    const int threads = cfg.block_x > 0 ? cfg.block_x : 12 * kWave;
+    const int threads = cfg.block_x > 0 ? cfg.block_x : (depth == 3 ? 8 : 12) * kWave;
     return m.nwd > 0 && m.nww > 0 && m.nwd * m.nww * kWave == threads && march2_supported(m, storage);
 }
 

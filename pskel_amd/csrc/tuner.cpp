@@ -242,7 +242,7 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
         // grids measured; at 277 .. 342 deep it lost by up to 5%)
         const double ntd3 = q.depth <= 512 ? 1.0 : 1.0 + std::ceil((q.depth - 512.0) / 504.0);
         if (q.storage == PSKEL_DTYPE_F32 && q.iterations >= 3 && (double)q.depth / (512.0 * ntd3) >= 0.7)
-            return fused(4, 2, 768, 2, 3);
+            return fused(4, 4, 512, 2, 3);
         if (q.iterations >= 2 && bf >= 0.2f) return pairs[bi];
         // one sweep per launch: a 64-lane row as wide as the depth needs, and
         // segments short enough for about 1024 blocks

@@ -68,8 +68,8 @@ def test_3d_heuristic_fits_the_depth():
     h = lambda shape, st: T.heuristic_choice(_q(shape, st, "jacobi3d", 100))  # noqa: E731
     key = lambda c: (c.time_tile, c.tile_x, c.tile_y, c.block_x, c.block_y)  # noqa: E731
     # f32: three sweeps per launch once 512-cell tiles are 70% full
-    assert key(h((512, 512, 512), "f32")) == (3, 4, 2, 768, 2)
-    assert key(h((479, 323, 406), "f32")) == (3, 4, 2, 768, 2)
+    assert key(h((512, 512, 512), "f32")) == (3, 4, 4, 512, 2)
+    assert key(h((479, 323, 406), "f32")) == (3, 4, 4, 512, 2)
     assert key(h((349, 507, 323), "f32"))[0] == 2
     assert key(h((410, 287, 517), "f32"))[0] == 2  # two 512-cell rows would be half empty
     assert key(h((512, 512, 512), "bf16")) == (2, 4, 3, 768, 2)
