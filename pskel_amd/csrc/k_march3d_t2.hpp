@@ -601,13 +601,35 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
         }
     };  // segment
 
-    // tiles fastest, then segments: blocks that run at once on one XCD hold
-    // neighbouring tiles at the same planes, so the rows they share are read
-    // from HBM once
+    // The block's work as a range [a, e) of (tile, plane) pairs in tile-major
+    // order (index tile * H + plane), marched as one segment per tile it
+    // touches.
+    // hseg > 0: one segment of hseg planes; tiles fastest, then segments, so
+    //   blocks that run at once on one XCD hold neighbouring tiles at the same
+    //   planes and the rows they share are read from HBM once.
+    // hseg < 0: the grid's tile-planes split evenly, -hseg per block, over
+    //   one block per resident slot: no second round of blocks.  A block's
+    //   range may end in one tile and go on in the next.  Neighbouring tiles'
+    //   blocks are consecutive in the XCD order and march at plane offsets
+    //   below the range length.
     const int tiles = ntd * ntw;
-    const int tile = (int)(lid % tiles), sg = (int)(lid / tiles);
-    const int h0 = sg * hseg;
-    segment(tile % ntd, tile / ntd, h0, min(h0 + hseg, H));
+    long long a, e;
+    if (hseg > 0) {
+        const int tile = (int)(lid % tiles), sg = (int)(lid / tiles);
+        a = (long long)tile * H + (long long)sg * hseg;
+        e = (long long)tile * H + min(sg * hseg + hseg, H);
+    } else {
+        const long long per = -(long long)hseg;
+        a = (long long)lid * per;
+        e = min(a + per, (long long)tiles * H);
+    }
+#pragma unroll 1
+    while (a < e) {
+        const int t = (int)(a / H), p0 = (int)(a % H);
+        const int p1 = (int)min((long long)H, (long long)p0 + (e - a));
+        segment(t % ntd, t / ntd, p0, p1);
+        a += p1 - p0;
+    }
 
     if constexpr (RES) {
         const double t = block_sum<NW>(rr, red);
@@ -615,11 +637,16 @@ __global__ __launch_bounds__(NWD * NWW * kWave) void k3d_march2(Sweep3D s, float
     }
 }
 
-// Blocks and h planes per block.  cfg.tile_z fixes the planes; by default
-// the segment count n maximises the useful fraction of the device's block
-// slots (CUs x resident blocks per CU) over the rounds the grid takes, net of
-// the 2*DEPTH planes a segment reads twice: (tiles * n / (rounds * slots)) *
-// (hseg / (hseg + 2*DEPTH)).
+// Blocks and h planes per block.  cfg.tile_z fixes the planes (segments of
+// that many planes, *hseg > 0).  By default two splits are scored by the
+// useful fraction of the device's block slots (CUs x resident blocks per CU),
+// net of the 2*DEPTH planes a segment reads twice:
+// * n equal segments per tile (*hseg > 0): (tiles * n / (rounds * slots)) *
+//   (hseg / (hseg + 2*DEPTH));
+// * the tiles' planes split evenly over at most one block per slot, at least
+//   4*DEPTH planes each (*hseg = -planes per block): (blocks / slots) *
+//   (per / (per + 2*DEPTH * tiles a range touches)).
+// The even split is taken when it scores 2% more.
 template <int ST, int VD, int RW, int NWD, int NWW, int PF, int DEPTH>
 size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
 {
@@ -666,6 +693,16 @@ size_t march2_grid(const Sweep3D& s, const March2Shape& m, int* hseg)
                 best = u;
                 q = hs;
             }
+        }
+        const int64_t total = tiles * (int64_t)s.H;
+        const int64_t minlen = 4 * DEPTH;
+        const int64_t nb2 = std::max<int64_t>(1, std::min<int64_t>(slots, (total + minlen - 1) / minlen));
+        const int64_t per = (total + nb2 - 1) / nb2;
+        const double touched = 1.0 + (double)(per - 1) / (double)s.H;  // tiles a range spans, on average
+        const double u2 = (double)nb2 / (double)slots * (double)per / ((double)per + 2.0 * DEPTH * touched);
+        if (u2 > best * 1.02 && per <= 0x7fffffff) {
+            *hseg = -(int)per;
+            return (size_t)((total + per - 1) / per);
         }
     }
     *hseg = q;

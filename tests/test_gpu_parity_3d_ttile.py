@@ -380,3 +380,30 @@ def test_automatic_storage_offset(gpu, store):
         g.run(K.jacobi3d(), 7, variant="tuned", time_tile=2)
         want = g.download(raw=True)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("tshape,depth,store", [((4, 2, 2, 6, 1), 3, "f32"), ((4, 3, 2, 6, 1), 2, "f32"),
+                                                ((4, 3, 2, 6, 1), 2, "bf16")])
+def test_fused_even_split_over_tiles(gpu, tshape, depth, store):
+    """Default planes per block on a grid whose tiles x planes split evenly over
+    one block per CU (at 256 CUs a block's range of 9..18 tile-planes often
+    ends in one tile and goes on in the next): bit-exact, with residuals at
+    launch ends, against the oracle."""
+    shape = (120, 300, 70)
+    a = _field(shape, 23)
+    vd, rw, nwd, nww, pf = tshape
+    kw = dict(variant="tuned", block=_block(nwd, nww, pf), tile=(vd, rw, 0), time_tile=depth, residual_every=3)
+    if store == "f32":
+        with api.Grid(shape, "f32") as g:
+            g.upload_pair(a)
+            st = g.run(K.jacobi3d(), 9, **kw)
+            got = g.download()
+        want, want_res = oracle.run_3d(K.jacobi3d(), a, 9, residual_every=3)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    else:
+        words = oracle.encode16(a, oracle.STORE_BF16, 0.5)
+        got, st = _run16(words, 9, store, tshape, 0, 0.5, "stochastic", seed=5, residual_every=3)
+        want, want_res = oracle.run_3d_r16(K.jacobi3d(), oracle.STORE_BF16, words, 9, store_offset=0.5,
+                                           rounding=oracle.ROUND_STOCHASTIC, seed=5, residual_every=3)
+        assert np.array_equal(got, want)
+    np.testing.assert_allclose(np.array(st["residuals"]), want_res, rtol=1e-12, atol=0)
