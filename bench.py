@@ -279,6 +279,32 @@ def measure_dropin(sweeps=1000, every=50):
             "residuals": len(res[-1].split()) - 1 if res else 0}
 
 
+def measure_transfers(shape=(512, 512, 512), reps=3):
+    """pskel_grid_upload / pskel_grid_download of a 512^3 f32 host array (the
+    drop-in's per-call copies) into f32 and bf16 storage: best of `reps` after
+    one warm call (which allocates the pinned staging), GB/s of host bytes."""
+    from pskel_amd.api import Grid
+
+    h = np.random.default_rng(0).standard_normal(shape, dtype=np.float32)
+    out = {}
+    for st in ("f32", "bf16"):
+        with Grid(shape, storage=st) as g:
+            up, down = [], []
+            for _ in range(reps + 1):
+                t0 = time.perf_counter()
+                g.upload(0, h)
+                t1 = time.perf_counter()
+                g.download(np.float32)
+                t2 = time.perf_counter()
+                up.append(t1 - t0)
+                down.append(t2 - t1)
+            u, d = min(up[1:]), min(down[1:])
+            out[st] = {"upload_ms": round(u * 1e3, 2), "download_ms": round(d * 1e3, 2),
+                       "upload_gbs": round(h.nbytes / u / 1e9, 1), "download_gbs": round(h.nbytes / d / 1e9, 1)}
+    out["host_bytes"] = h.nbytes
+    return out
+
+
 def host_search(f, s, sweeps):
     """(slab height, sweeps per pass) from pskel_tuner_host_search."""
     import ctypes as C
@@ -755,6 +781,11 @@ def main(argv=None):
         if rd and "gpu_ms" in rd:
             print(f"# jacobi3d drop-in runIterativeGPU(1000): {rd['cell_updates_per_s_device']:.4g} CU/s device, "
                   f"{rd['cell_updates_per_s_with_pcie']:.4g} CU/s with PCIe", file=sys.stderr, flush=True)
+        rt = measure_transfers()
+        extra["jacobi3d:host_transfers"] = rt
+        print("# host transfers 512^3 f32 host array: " + ", ".join(
+            f"{st} up {rt[st]['upload_gbs']} GB/s down {rt[st]['download_gbs']} GB/s" for st in ("f32", "bf16")),
+              file=sys.stderr, flush=True)
         # the tuner: its pick for each workload against exhaustive search
         tuner = {}
         for nm in TUNER_WORKLOADS:

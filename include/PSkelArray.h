@@ -141,6 +141,10 @@ public:
                 for (size_t k = 0; k < depth; ++k) hostGet(i, j, k) = array.hostGet(i, j, k);
     }
 
+    // The host array when this handle covers all of it in the packed order (no
+    // slice), else NULL: the GPU runners move it without packing.
+    T* denseHost() const { return size() == realSize() ? hostArray : nullptr; }
+
     // Pack the virtual array (a slice or not) into a dense buffer and back.
     void packTo(T* dst) const
     {

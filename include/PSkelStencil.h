@@ -499,18 +499,28 @@ protected:
             gd.depth = dims() == 3 ? input.getDepth() : 1;
             gd.device = s.device;
             pskel_stencil_desc_t desc = StencilTraits<Args>::describe(mask, args, dims());
-            std::vector<T> in(input.size()), out(output.size());
-            input.packTo(in.data());
-            output.packTo(out.data());
+            // whole arrays are read and written in place, slices through packed copies
+            std::vector<T> in, out;
+            const T* inHost = input.denseHost();
+            T* outHost = output.denseHost();
+            if (!inHost) {
+                in.resize(input.size());
+                input.packTo(in.data());
+                inHost = in.data();
+            }
+            if (!outHost) {
+                out.resize(output.size());
+                output.packTo(out.data());
+                outHost = out.data();
+            }
             pskel_launch_cfg_t cfg{};
             cfg.variant = PSKEL_VARIANT_TUNED;
             if (s.haveCfg) cfg = s.cfg;
             cfg.residual_every = 0;
             pskel_run_stats_t st{};
-            gpuErrchk(pskel_run_iterative_host(&gd, in.data(), out.data(), hdt, &desc, iterations, tile, inner, &cfg,
-                                               &st));
+            gpuErrchk(pskel_run_iterative_host(&gd, inHost, outHost, hdt, &desc, iterations, tile, inner, &cfg, &st));
             s.lastMs = st.elapsed_ms;
-            output.unpackFrom(out.data());
+            if (!out.empty()) output.unpackFrom(out.data());
         }
     }
 
@@ -522,12 +532,23 @@ protected:
         const int hdt = detail::hostDtype<T>();
         if (hdt == 0) gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);
         // upload both buffers: the device state starts as the host state
-        std::vector<T> buf(input.size()), obuf(output.size());
-        input.packTo(buf.data());
-        output.packTo(obuf.data());
+        // (whole arrays go from their own host memory, slices are packed first)
+        std::vector<T> buf, obuf;
+        const T* inHost = input.denseHost();
+        const T* outHost = output.denseHost();
+        if (!inHost) {
+            buf.resize(input.size());
+            input.packTo(buf.data());
+            inHost = buf.data();
+        }
+        if (!outHost) {
+            obuf.resize(output.size());
+            output.packTo(obuf.data());
+            outHost = obuf.data();
+        }
         auto upload = [&]() {
-            gpuErrchk(pskel_grid_upload(s.grid, PSKEL_BUFFER_INPUT, buf.data(), hdt));
-            gpuErrchk(pskel_grid_upload(s.grid, PSKEL_BUFFER_OUTPUT, obuf.data(), hdt));
+            gpuErrchk(pskel_grid_upload(s.grid, PSKEL_BUFFER_INPUT, inHost, hdt));
+            gpuErrchk(pskel_grid_upload(s.grid, PSKEL_BUFFER_OUTPUT, outHost, hdt));
         };
         upload();
         if (!plain && s.haveCfg) cfg = s.cfg;
@@ -581,8 +602,13 @@ protected:
             gpuErrchk((pskel_status_t)PSKEL_ERR_UNSUPPORTED);  // compile the TU with hipcc or describe the kernel
 #endif
         }
-        gpuErrchk(pskel_grid_download(s.grid, buf.data(), hdt));
-        output.unpackFrom(buf.data());
+        if (T* dst = output.denseHost()) {
+            gpuErrchk(pskel_grid_download(s.grid, dst, hdt));
+        } else {
+            obuf.resize(output.size());
+            gpuErrchk(pskel_grid_download(s.grid, obuf.data(), hdt));
+            output.unpackFrom(obuf.data());
+        }
 #ifdef PSKEL_PRINT_TIMES
         std::cout << "GPU_time\t" << s.lastMs * 1e-3 << std::endl;
 #endif

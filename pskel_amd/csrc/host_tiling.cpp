@@ -56,26 +56,6 @@ int storage_bytes(int dtype)
     }
 }
 
-// large host copies on several threads (pinned <-> pageable)
-void par_copy(void* dst, const void* src, size_t bytes)
-{
-    const size_t chunk = size_t(32) << 20;
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), (bytes + chunk - 1) / chunk);
-    if (nt <= 1) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    std::vector<std::thread> th;
-    const size_t per = (bytes + nt - 1) / nt;
-    for (size_t t = 0; t < nt; ++t) {
-        const size_t a = t * per, b = std::min(bytes, a + per);
-        if (a >= b) break;
-        th.emplace_back([=]() { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
-    }
-    for (auto& x : th) x.join();
-}
-
 struct Slab {
     size_t h0, h1;  // core rows/planes
     size_t s0, s1;  // staged rows/planes (core plus halo, clamped)

@@ -231,50 +231,122 @@ Sweep3D make_sweep3d(const pskel_grid_t* g, const void* src, void* dst, int radi
     return s;
 }
 
-// host rows <-> device pitched rows
-pskel_status_t copy_h2d(pskel_grid_t* g, void* dbuf, const void* host)
-{
-    const size_t row_bytes = (g->ndim == 2 ? g->W : g->D) * g->elem;
-    PSKEL_HIP(hipMemcpy2DAsync(dbuf, g->pitch * g->elem, host, row_bytes, row_bytes, g->nrows,
-                               hipMemcpyHostToDevice, g->stream));
-    PSKEL_HIP(hipStreamSynchronize(g->stream));
-    return PSKEL_OK;
-}
-
-pskel_status_t copy_d2h(pskel_grid_t* g, void* host, const void* dbuf)
-{
-    const size_t row_bytes = (g->ndim == 2 ? g->W : g->D) * g->elem;
-    PSKEL_HIP(hipMemcpy2DAsync(host, row_bytes, dbuf, g->pitch * g->elem, row_bytes, g->nrows,
-                               hipMemcpyDeviceToHost, g->stream));
-    PSKEL_HIP(hipStreamSynchronize(g->stream));
-    return PSKEL_OK;
-}
+// ---------------------------------------------------------------------------
+// Host rows <-> device pitched rows.  Transfers go through two pinned staging
+// buffers used in turn: the host side of one chunk of rows (a copy or a storage
+// conversion, on several threads) overlaps the DMA of the previous chunk.
+// A pageable source would be copied by the runtime one pitched row at a time.
+// ---------------------------------------------------------------------------
+constexpr size_t kStageBytes = size_t(16) << 20;
 
 size_t cells(const pskel_grid_t* g) { return g->W * g->H * g->D; }
+size_t row_cells(const pskel_grid_t* g) { return g->ndim == 2 ? g->W : g->D; }
+
+pskel_status_t ensure_stage(pskel_grid_t* g)
+{
+    const size_t rb = row_cells(g) * g->elem;
+    const size_t want = std::max(rb, std::min(kStageBytes, rb * g->nrows));
+    if (g->stage_bytes >= want) return PSKEL_OK;
+    PSKEL_HIP(hipStreamSynchronize(g->stream));
+    for (int b = 0; b < 2; ++b) {
+        if (g->stage[b]) (void)hipHostFree(g->stage[b]);
+        g->stage[b] = nullptr;
+    }
+    g->stage_bytes = 0;
+    for (int b = 0; b < 2; ++b) {
+        PSKEL_HIP(hipHostMalloc(&g->stage[b], want, hipHostMallocDefault));
+        if (!g->stage_ev[b]) PSKEL_HIP(hipEventCreateWithFlags(&g->stage_ev[b], hipEventDisableTiming));
+    }
+    g->stage_bytes = want;
+    return PSKEL_OK;
+}
+
+// rows [r0, r0 + n) between a staging buffer (packed rows) and device memory
+hipError_t stage_copy(pskel_grid_t* g, void* dbuf, void* stage, size_t r0, size_t n, hipMemcpyKind kind)
+{
+    const size_t rb = row_cells(g) * g->elem, pb = g->pitch * g->elem;
+    char* dev = (char*)dbuf + r0 * pb;
+    if (kind == hipMemcpyHostToDevice)
+        return pb == rb ? hipMemcpyAsync(dev, stage, n * rb, kind, g->stream)
+                        : hipMemcpy2DAsync(dev, pb, stage, rb, rb, n, kind, g->stream);
+    return pb == rb ? hipMemcpyAsync(stage, dev, n * rb, kind, g->stream)
+                    : hipMemcpy2DAsync(stage, rb, dev, pb, rb, n, kind, g->stream);
+}
+
+// fill(stage, first_cell, ncells) packs host cells into the staging buffer
+template <class Fill>
+pskel_status_t upload_rows(pskel_grid_t* g, void* dbuf, Fill fill)
+{
+    pskel_status_t st = ensure_stage(g);
+    if (st != PSKEL_OK) return st;
+    const size_t rc = row_cells(g), per = g->stage_bytes / (rc * g->elem);
+    size_t i = 0;
+    for (size_t r0 = 0; r0 < g->nrows; r0 += per, ++i) {
+        const size_t n = std::min(per, g->nrows - r0);
+        const int b = (int)(i & 1);
+        if (i >= 2) PSKEL_HIP(hipEventSynchronize(g->stage_ev[b]));  // its previous chunk has left
+        fill((unsigned char*)g->stage[b], r0 * rc, n * rc);
+        PSKEL_HIP(stage_copy(g, dbuf, g->stage[b], r0, n, hipMemcpyHostToDevice));
+        PSKEL_HIP(hipEventRecord(g->stage_ev[b], g->stream));
+    }
+    PSKEL_HIP(hipStreamSynchronize(g->stream));
+    return PSKEL_OK;
+}
+
+// drain(stage, first_cell, ncells) unpacks the staging buffer into host cells
+template <class Drain>
+pskel_status_t download_rows(pskel_grid_t* g, const void* dbuf, Drain drain)
+{
+    pskel_status_t st = ensure_stage(g);
+    if (st != PSKEL_OK) return st;
+    const size_t rc = row_cells(g), per = g->stage_bytes / (rc * g->elem);
+    const size_t chunks = (g->nrows + per - 1) / per;
+    auto issue = [&](size_t i) -> hipError_t {
+        const size_t r0 = i * per;
+        hipError_t e = stage_copy(g, (void*)dbuf, g->stage[i & 1], r0, std::min(per, g->nrows - r0),
+                                  hipMemcpyDeviceToHost);
+        return e == hipSuccess ? hipEventRecord(g->stage_ev[i & 1], g->stream) : e;
+    };
+    if (chunks) PSKEL_HIP(issue(0));
+    for (size_t i = 0; i < chunks; ++i) {
+        if (i + 1 < chunks) PSKEL_HIP(issue(i + 1));  // into the buffer drained last time round
+        PSKEL_HIP(hipEventSynchronize(g->stage_ev[i & 1]));
+        const size_t r0 = i * per;
+        drain((const unsigned char*)g->stage[i & 1], r0 * rc, std::min(per, g->nrows - r0) * rc);
+    }
+    return PSKEL_OK;
+}
+
+constexpr size_t kConvertGrain = size_t(256) << 10;  // cells per thread, at least
 
 pskel_status_t download_buf(pskel_grid_t* g, const void* dbuf, void* host, int32_t host_dtype)
 {
-    const size_t n = cells(g);
-    if (host_dtype == g->storage) return copy_d2h(g, host, dbuf);
+    if (host_dtype == g->storage) {
+        const size_t e = g->elem;
+        return download_rows(g, dbuf, [&](const unsigned char* sb, size_t c0, size_t n) {
+            par_copy((unsigned char*)host + c0 * e, sb, n * e);
+        });
+    }
     if (g->storage == PSKEL_DTYPE_U8) return fail(PSKEL_ERR_INVALID_ARGUMENT, "u8 grid: host dtype must be U8");
     if (host_dtype != PSKEL_DTYPE_F32 && host_dtype != PSKEL_DTYPE_F64)
         return fail(PSKEL_ERR_INVALID_ARGUMENT, "host dtype must be the storage type, F32 or F64");
-    std::vector<unsigned char> tmp(n * g->elem);
-    pskel_status_t st = copy_d2h(g, tmp.data(), dbuf);
-    if (st != PSKEL_OK) return st;
     const float off = g->store_offset;
-    for (size_t i = 0; i < n; ++i) {
-        float v;
-        if (g->storage == PSKEL_DTYPE_F32) std::memcpy(&v, &tmp[i * 4], 4);
-        else {
-            uint16_t h;
-            std::memcpy(&h, &tmp[i * 2], 2);
-            v = (g->storage == PSKEL_DTYPE_BF16 ? bf16_to_f32(h) : f16_to_f32(h)) + off;
-        }
-        if (host_dtype == PSKEL_DTYPE_F32) ((float*)host)[i] = v;
-        else ((double*)host)[i] = (double)v;
-    }
-    return PSKEL_OK;
+    const int storage = g->storage;
+    return download_rows(g, dbuf, [&](const unsigned char* sb, size_t c0, size_t n) {
+        par_range(n, kConvertGrain, [&](size_t a, size_t b) {
+            for (size_t i = a; i < b; ++i) {
+                float v;
+                if (storage == PSKEL_DTYPE_F32) std::memcpy(&v, sb + i * 4, 4);
+                else {
+                    uint16_t h;
+                    std::memcpy(&h, sb + i * 2, 2);
+                    v = (storage == PSKEL_DTYPE_BF16 ? bf16_to_f32(h) : f16_to_f32(h)) + off;
+                }
+                if (host_dtype == PSKEL_DTYPE_F32) ((float*)host)[c0 + i] = v;
+                else ((double*)host)[c0 + i] = (double)v;
+            }
+        });
+    });
 }
 
 pskel_status_t ensure_partials(pskel_grid_t* g, size_t n)
@@ -746,6 +818,10 @@ pskel_status_t pskel_grid_destroy(pskel_grid_t* g)
         if (ge.exec) (void)hipGraphExecDestroy(ge.exec);
     for (int i = 0; i < 3; ++i)
         if (g->buf[i]) (void)hipFree(g->buf[i]);
+    for (int b = 0; b < 2; ++b) {
+        if (g->stage[b]) (void)hipHostFree(g->stage[b]);
+        if (g->stage_ev[b]) (void)hipEventDestroy(g->stage_ev[b]);
+    }
     if (g->d_partials) (void)hipFree(g->d_partials);
     if (g->d_res) (void)hipFree(g->d_res);
     if (g->ev0) (void)hipEventDestroy(g->ev0);
@@ -779,7 +855,10 @@ pskel_status_t pskel_grid_upload(pskel_grid_t* g, int32_t which, const void* hos
     if (host_dtype == g->storage) {
         if (g->auto_offset && which == PSKEL_BUFFER_INPUT)
             return fail(PSKEL_ERR_INVALID_ARGUMENT, "an automatic storage offset is the mean of F32 or F64 data");
-        return copy_h2d(g, dbuf, host);
+        const size_t e = g->elem;
+        return upload_rows(g, dbuf, [&](unsigned char* sb, size_t c0, size_t m) {
+            par_copy(sb, (const unsigned char*)host + c0 * e, m * e);
+        });
     }
     if (host_dtype != PSKEL_DTYPE_F32 && host_dtype != PSKEL_DTYPE_F64)
         return fail(PSKEL_ERR_INVALID_ARGUMENT, "host dtype must be the storage type, F32 or F64");
@@ -796,17 +875,21 @@ pskel_status_t pskel_grid_upload(pskel_grid_t* g, int32_t which, const void* hos
         }
         g->store_offset = cnt ? (float)(sum / (double)cnt) : 0.0f;
     }
-    std::vector<unsigned char> tmp(n * g->elem);
     const float off = g->store_offset;
-    for (size_t i = 0; i < n; ++i) {
-        const float v = host_dtype == PSKEL_DTYPE_F32 ? ((const float*)host)[i] : (float)((const double*)host)[i];
-        if (g->storage == PSKEL_DTYPE_F32) std::memcpy(&tmp[i * 4], &v, 4);
-        else {
-            const uint16_t h = g->storage == PSKEL_DTYPE_BF16 ? f32_to_bf16(v - off) : f32_to_f16(v - off);
-            std::memcpy(&tmp[i * 2], &h, 2);
-        }
-    }
-    return copy_h2d(g, dbuf, tmp.data());
+    const int storage = g->storage;
+    return upload_rows(g, dbuf, [&](unsigned char* sb, size_t c0, size_t m) {
+        par_range(m, kConvertGrain, [&](size_t a, size_t b) {
+            for (size_t i = a; i < b; ++i) {
+                const size_t c = c0 + i;
+                const float v = host_dtype == PSKEL_DTYPE_F32 ? ((const float*)host)[c] : (float)((const double*)host)[c];
+                if (storage == PSKEL_DTYPE_F32) std::memcpy(sb + i * 4, &v, 4);
+                else {
+                    const uint16_t h = storage == PSKEL_DTYPE_BF16 ? f32_to_bf16(v - off) : f32_to_f16(v - off);
+                    std::memcpy(sb + i * 2, &h, 2);
+                }
+            }
+        });
+    });
 }
 
 pskel_status_t pskel_grid_download(pskel_grid_t* g, void* host, int32_t host_dtype)

@@ -4,9 +4,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pskel_amd.h"
@@ -19,6 +22,35 @@ namespace pskel {
 void set_error(const std::string& msg);
 pskel_status_t fail(pskel_status_t st, const std::string& msg);
 pskel_status_t hip_fail(hipError_t e, const char* what);
+
+// ---------------------------------------------------------------------------
+// Host-side data movement on several threads (at most 16).
+// ---------------------------------------------------------------------------
+// f(a, b) over disjoint ranges covering [0, n), pieces of at least `grain`.
+template <class F>
+void par_range(size_t n, size_t grain, F f)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), (n + grain - 1) / grain);
+    if (nt <= 1) {
+        if (n) f(size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    for (size_t t = 0; t < nt; ++t) {
+        const size_t a = t * per, b = std::min(n, a + per);
+        if (a >= b) break;
+        th.emplace_back(f, a, b);
+    }
+    for (auto& x : th) x.join();
+}
+
+inline void par_copy(void* dst, const void* src, size_t bytes)
+{
+    par_range(bytes, size_t(2) << 20,
+              [=](size_t a, size_t b) { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
+}
 
 #define PSKEL_HIP(call)                                                        \
     do {                                                                       \
@@ -203,6 +235,11 @@ struct GraphEntry {
 
 struct pskel_grid : pskel::Grid {
     std::vector<GraphEntry> graphs;
+    // pinned staging for uploads and downloads: two buffers, filled and sent
+    // in turn (allocated by the first transfer, at most 16 MiB each)
+    void* stage[2] = {nullptr, nullptr};
+    size_t stage_bytes = 0;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 namespace pskel {
