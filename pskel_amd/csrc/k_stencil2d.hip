@@ -11,7 +11,11 @@
 //              exchanged across lanes, all row loads issued before any math.
 //   k2d_ttile  overlapped temporal tile: K sweeps in LDS per launch, K-cell halo
 //              recomputed redundantly (the rewrite of PSkelStencilTiling).
+//   k2d_march  K sweeps per launch pipelined in registers down a segment of
+//              rows, DPP neighbour exchange, no LDS (k_march2d.hpp).
 #include "k_tiles2d.hpp"
+#define PSKEL_MARCH2D_INSTANTIATE
+#include "k_march2d.hpp"
 
 namespace pskel {
 
@@ -148,7 +152,36 @@ hipError_t ttile_dispatch(const Sweep2D& s, const Points& p, int k, int cy, hipS
     }
 }
 
+template <int PAT, int PD>
+hipError_t march2d_dispatch(const Sweep2D& s, const Points& p, int k, int seg, hipStream_t st)
+{
+    if (!march2d_fits(s.W, s.H, s.pitch)) return hipErrorInvalidValue;
+    const March2DShape m = march2d_shape(s.W, s.H, seg);
+    if (m.units == 0) return hipSuccess;  // no interior cells: a sweep writes nothing
+    const dim3 grid((unsigned)((m.units + kMarch2DWaves - 1) / kMarch2DWaves)), block(kMarch2DWaves * kWave);
+    switch (k) {
+    case 2: hipLaunchKernelGGL((k2d_march<PAT, 2, PD>), grid, block, 0, st, s, p, m.ntw, m.seg, m.units); break;
+    case 3: hipLaunchKernelGGL((k2d_march<PAT, 3, PD>), grid, block, 0, st, s, p, m.ntw, m.seg, m.units); break;
+    case 4: hipLaunchKernelGGL((k2d_march<PAT, 4, PD>), grid, block, 0, st, s, p, m.ntw, m.seg, m.units); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
+
+bool march2d_ok(int W, int H, size_t pitch) { return march2d_fits(W, H, pitch); }
+
+hipError_t launch_2d_march_f32(const Sweep2D& s, const Points& p, int pattern, int k,
+                               const pskel_launch_cfg_t& cfg, hipStream_t st)
+{
+    if (cfg.tile_x != kMarch2DVd || cfg.tile_y != 0 || cfg.tile_z < 0) return hipErrorInvalidValue;
+    // input rows loaded 3 ahead: deeper prefetch measured no faster (the
+    // 9-point march is bound by VALU issue, the 5-point one by its short waves)
+    if (pattern == PAT_J5) return march2d_dispatch<PAT_J5, 3>(s, p, k, cfg.tile_z, st);
+    if (pattern == PAT_BOX9) return march2d_dispatch<PAT_BOX9, 3>(s, p, k, cfg.tile_z, st);
+    return hipErrorInvalidValue;
+}
 
 hipError_t launch_2d_plain_f32(const Sweep2D& s, const Points& p, int bx, int by, hipStream_t st)
 {

@@ -485,7 +485,9 @@ pskel_status_t issue(pskel_grid_t* g, const pskel_stencil_desc_t& d, const Point
                 else if (plan.path >= 1) e = launch_life_tuned(s, one, st);
                 else e = launch_life_plain(s, pts, bx, by, st);
             } else {
-                if (k > 1) e = launch_2d_ttile_f32(s, pts, plan.pattern, (int)k, cfg, st);
+                if (k > 1)
+                    e = cfg.tile_x > 0 ? launch_2d_march_f32(s, pts, plan.pattern, (int)k, cfg, st)
+                                       : launch_2d_ttile_f32(s, pts, plan.pattern, (int)k, cfg, st);
                 else if (plan.path >= 1) e = launch_2d_tuned_f32(s, pts, plan.pattern, one, st);
                 else e = launch_2d_plain_f32(s, pts, bx, by, st);
             }
@@ -601,6 +603,13 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
             // life tiles keep a ring of one cell: a larger mask range sweeps
             // one generation per launch.
             plan.path = (plan.pattern == PAT_LIFE8 && radius != 1) ? 1 : 2;
+            // tile_x > 0 selects the register march (4 cells per lane) over the LDS tile
+            if (plan.pattern != PAT_LIFE8 && cfg.tile_x != 0) {
+                if (cfg.tile_x != 4 || cfg.tile_y != 0 || cfg.tile_z < 0)
+                    return fail(PSKEL_ERR_UNSUPPORTED, "2D register march: tile (4, 0, rows per segment >= 0)");
+                if (!march2d_ok((int)g->W, (int)g->H, g->pitch))
+                    return fail(PSKEL_ERR_UNSUPPORTED, "2D register march: grids wider than 8 cells, under 2 GiB");
+            }
         }
         if (cfg.time_tile > 3 && ndim == 3) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 3");
         // two or three fused sweeps per launch run the 7-point Jacobi; for any

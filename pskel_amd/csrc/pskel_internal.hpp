@@ -193,6 +193,12 @@ hipError_t launch_2d_tuned_f32(const Sweep2D& s, const Points& p, int pattern,
                                const pskel_launch_cfg_t& cfg, hipStream_t st);
 hipError_t launch_2d_ttile_f32(const Sweep2D& s, const Points& p, int pattern, int k,
                                const pskel_launch_cfg_t& cfg, hipStream_t st);
+// k fused sweeps (2..4) marched in registers: cfg.tile_x = 4 cells per lane,
+// cfg.tile_z = rows per segment (0: automatic)
+hipError_t launch_2d_march_f32(const Sweep2D& s, const Points& p, int pattern, int k,
+                               const pskel_launch_cfg_t& cfg, hipStream_t st);
+// a grid the register march takes: wider than 8 cells, buffer under 2 GiB
+bool march2d_ok(int W, int H, size_t pitch);
 
 // 2D life (u8) sweeps.
 hipError_t launch_life_plain(const Sweep2D& s, const Points& p, int bx, int by, hipStream_t st);

@@ -84,6 +84,14 @@ pskel_launch_cfg_t fused(int vd, int rw, int threads, int nwd, int depth = 2)
     return c;
 }
 
+// the grid a 2D query describes fits the register march (its row pitch is a
+// whole number of 1 KiB, as pskel_grid_create lays it out)
+bool march2d_fits_query(const pskel_tune_query_t& q)
+{
+    const size_t pitch = (q.width * 4 + 1023) / 1024 * 1024 / 4;
+    return q.ndim == 2 && march2d_ok((int)q.width, (int)q.height, pitch);
+}
+
 pskel_launch_cfg_t plain(int bx, int by)
 {
     pskel_launch_cfg_t c{};
@@ -106,6 +114,10 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
         } else if (pat != PAT_CLOUD) {  // field stencils run one sweep per launch
             for (int k = 2; k <= 4; ++k)
                 for (int cy : {8, 16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
+            // the register march: rows per segment (0: about 3 waves per SIMD)
+            if (march2d_fits_query(q))
+                for (int k = 2; k <= 4; ++k)
+                    for (int seg : {0, 8, 24, 64}) v.push_back(tuned(4, 0, seg, k));
         }
         v.push_back(plain(64, 4));
     } else if (q.ndim == 3 && pat == PAT_J7) {
@@ -219,9 +231,16 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
         return tuned(0, 1, 4, 1);
     }
     if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9)) {
-        // temporal tiles of 16 rows pay at every grid size measured (64 .. 8192 a side)
-        if (q.iterations >= 2) return tuned(0, 16, 0, (int)std::min<size_t>(q.iterations, 4));
-        return tuned(0, 2, 4, 1);
+        // from 3M cells the register march (4 sweeps, automatic segments):
+        // 1.2-1.8x the LDS tiles on every such grid measured; below, the LDS
+        // tiles, which keep more waves busy: 16 rows and 4 sweeps from 1M
+        // cells, 8 rows and 3 sweeps under it (the march there: 1.5-2x slower)
+        const double cells = (double)q.width * q.height;
+        if (q.iterations < 2) return tuned(0, 2, 4, 1);
+        if (cells >= 3.0 * (1 << 20) && march2d_fits_query(q))
+            return tuned(4, 0, 0, (int)std::min<size_t>(q.iterations, 4));
+        if (cells >= (double)(1 << 20)) return tuned(0, 16, 0, (int)std::min<size_t>(q.iterations, 4));
+        return tuned(0, 8, 0, (int)std::min<size_t>(q.iterations, 3));
     }
     if (q.ndim == 3 && pat == PAT_J7) {
         // the fused pair whose tiles the grid fills best (512-, 256- or

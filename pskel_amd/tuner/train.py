@@ -55,7 +55,8 @@ def main(argv=None):
         test_rows, miss_t = rows_from_json(saved["test"])
         # candidates added since the table was measured: time them now
         for table, missing in ((rows, miss), (test_rows, miss_t)):
-            for q in dict.fromkeys(q for q, _ in missing):
+            todo = list(dict.fromkeys(q for q, _ in missing))
+            for n, q in enumerate(todo):
                 m = Measurer(q)
                 try:
                     keys = [cfg_key(c) for c in candidates(q)]
@@ -64,6 +65,8 @@ def main(argv=None):
                             table.append({"q": q, "ci": keys.index(cfg_key(c)), "ms": m(c)})
                 finally:
                     m.close()
+                if n % 10 == 9 or n + 1 == len(todo):
+                    log(f"measured new candidates on {n + 1}/{len(todo)} queries ({time.time() - t0:.0f} s)")
         log(f"loaded {len(rows)} + {len(test_rows)} timings, measured {len(miss) + len(miss_t)} new candidates")
         train_q = list(dict.fromkeys(r["q"] for r in rows))
         test_q = list(dict.fromkeys(r["q"] for r in test_rows))

@@ -10,6 +10,17 @@ def _q(shape, storage, name, it=100):
     return T.Query(tuple(shape), storage, name, it)
 
 
+def test_2d_march_candidates():
+    """The register march joins the 2D Jacobi and 9-point candidates (3 depths
+    x 4 segment lengths) on grids wider than 8 cells, not life's."""
+    march = lambda q: [c for c in T.candidates(q) if c.tile_x == 4 and c.time_tile >= 2]  # noqa: E731
+    for name in ("jacobi2d", "diffusion9"):
+        m = march(_q((300, 700), "f32", name))
+        assert sorted((c.time_tile, c.tile_z) for c in m) == [(k, s) for k in (2, 3, 4) for s in (0, 8, 24, 64)]
+        assert march(_q((300, 8), "f32", name)) == []
+    assert march(_q((300, 700), "u8", "life")) == []
+
+
 @pytest.mark.parametrize("q,expect_tuned", [
     (_q((2048, 2048), "f32", "jacobi2d"), True),
     (_q((8194, 8194), "u8", "life"), True),
@@ -83,14 +94,17 @@ def test_3d_heuristic_fits_the_depth():
 
 
 def test_2d_heuristic():
-    """Temporal tiles of 16 rows for the Jacobi and 9-point stencils at every
-    size (depth 4, or the sweep count when smaller); life: the plain kernel on
-    small grids, bit tiles on wide ones."""
-    c = T.heuristic_choice(_q((125, 205), "f32", "jacobi2d", 158))
-    assert (c.tile_y, c.time_tile) == (16, 4)
-    c = T.heuristic_choice(_q((1310, 2751), "f32", "diffusion9", 324))
-    assert (c.tile_y, c.time_tile) == (16, 4)
-    assert T.heuristic_choice(_q((2903, 1983), "f32", "diffusion9", 3)).time_tile == 3
+    """Jacobi and 9-point stencils: the register march from 3M cells, LDS
+    temporal tiles of 16 rows from 1M cells and of 8 rows (3 sweeps) below
+    (depth 4, or the sweep count when smaller); life: the plain kernel on small
+    grids, bit tiles on wide ones."""
+    key = lambda c: (c.tile_x, c.tile_y, c.tile_z, c.time_tile)  # noqa: E731
+    assert key(T.heuristic_choice(_q((125, 205), "f32", "jacobi2d", 158))) == (0, 8, 0, 3)
+    assert key(T.heuristic_choice(_q((1024, 1331), "f32", "diffusion9", 29))) == (0, 16, 0, 4)
+    assert key(T.heuristic_choice(_q((1310, 2751), "f32", "diffusion9", 324))) == (4, 0, 0, 4)
+    assert key(T.heuristic_choice(_q((2903, 1983), "f32", "diffusion9", 3))) == (4, 0, 0, 3)
+    assert key(T.heuristic_choice(_q((2048, 2048), "f32", "jacobi2d", 2))) == (4, 0, 0, 2)
+    assert key(T.heuristic_choice(_q((300, 700), "f32", "jacobi2d", 2))) == (0, 8, 0, 2)
     assert T.heuristic_choice(_q((229, 71), "u8", "life", 107)).variant == 0
     assert T.heuristic_choice(_q((8194, 8194), "u8", "life", 500)).time_tile == 8
     assert T.heuristic_choice(_q((1683, 1653), "u8", "life", 29)).time_tile == 1
