@@ -647,9 +647,26 @@ PROFILES = {
     ("jacobi3d_bf16", "tuned"): "k3d_march2_jacobi3d_512_bf16_stochastic.txt",
     ("jacobi3d_bf16_single", "tuned"): "k3d_march_jacobi3d_512_bf16_stochastic.txt",
     ("life", "tuned"): "k_life_bits_life_8192_u8.txt",
-    ("jacobi2d", "tuned"): "k2d_ttile_jacobi2d_2048_f32.txt",
-    ("diffusion9", "tuned"): "k2d_ttile_diffusion9_4096_f32.txt",
+    ("jacobi2d", "tuned"): "k2d_march_jacobi2d_2048_f32.txt",
+    ("diffusion9", "tuned"): "k2d_march_diffusion9_4096_f32.txt",
 }
+
+
+def kernel_name(name, variant, launch):
+    """The kernel a workload's launch runs (tuned variant)."""
+    if variant != "tuned":
+        return None
+    k = launch.get("time_tile", 1)
+    tile = tuple(launch.get("tile", (0, 0, 0)))
+    if name.startswith("jacobi3d"):
+        return f"k3d_march2 (depth {k})" if k >= 2 else "k3d_march"
+    if name == "life":
+        return f"k_life_bits (depth {k})" if k >= 2 else "k_life_tuned"
+    if name in ("jacobi2d", "diffusion9"):
+        if k >= 2:
+            return f"k2d_march (depth {k})" if tile[0] == 4 else f"k2d_ttile (depth {k})"
+        return "k2d_tuned"
+    return None
 
 
 def traffic_from_profiles(name, variant="tuned"):
@@ -809,9 +826,7 @@ def main(argv=None):
         "traffic": traffic_from_profiles(args.workload, args.variant),
         "profile": (f"profiles/{PROFILES[(args.workload, args.variant)]}"
                     if (args.workload, args.variant) in PROFILES else None),
-        "kernel": ((f"k3d_march2 (depth {head['launch'].get('time_tile', 1)})"
-                    if head["launch"].get("time_tile", 1) >= 2 else "k3d_march")
-                   if args.workload.startswith("jacobi3d") and args.variant == "tuned" else None),
+        "kernel": kernel_name(args.workload, args.variant, head["launch"]),
         "sweeps_per_launch": round(head["sweeps_per_launch"], 3),
         "algorithmic_bytes_per_launch": bytes_per_launch,
         "algorithmic_bytes": (f"{head['algorithmic_bytes_per_cu']} B per cell (one read + one write of the grid) "

@@ -109,9 +109,12 @@ typedef struct pskel_launch_cfg {
     int32_t tile_y;         /* tuned: rows per lane (2D) / w-rows per wave (3D) */
     int32_t tile_z;         /* tuned 3D: planes per block segment along h; 2D: waves per block */
     int32_t time_tile;      /* tuned: sweeps fused per launch (temporal depth k); 0/1 = none.
-                               2D: k <= 4 (8 for life); 3D 7-point Jacobi: 2 (any storage) or
-                               3 (f32: tile (4,4,0) block (512,2), the default, or tile (4,2,0)
-                               block (768,2)); a launch never passes a residual step */
+                               2D: k <= 4 (8 for life): tile (0, CY, 0) LDS tiles of CY in
+                               {8,16,32,64} rows (0: 32), or tile (4, 0, S) the register march,
+                               S rows per segment (0: automatic; grids wider than 8 cells);
+                               3D 7-point Jacobi: 2 (any storage) or 3 (f32: tile (4,4,0) block
+                               (512,2), the default, or tile (4,2,0) block (768,2)); a launch
+                               never passes a residual step */
     int32_t residual_every; /* FP64 residual ||u^k - u^(k-1)||_2 every n sweeps; 0 = never */
     int32_t use_graph;      /* capture the launch sequence in a hipGraph and replay it once */
     int32_t reserved[3];
