@@ -29,7 +29,10 @@
 //
 // Memory.  Input rows come through buffer loads PD rows ahead; rows
 // outside the grid get an offset past the buffer (the range check returns
-// zeros).  Waves are independent: no barriers, no LDS.
+// zeros).  Every step issues the same loads and one store (out of range
+// while the last sweep has no output row yet), so the wait counts the
+// compiler derives hold on every path and the prefetch is never drained.
+// Waves are independent: no barriers, no LDS.
 #pragma once
 
 #include <type_traits>
@@ -183,17 +186,25 @@ __global__ __launch_bounds__(kMarch2DWaves * kWave) void k2d_march(Sweep2D s, Po
     float lv[NL][3][VD];      // sweeps 1..K-1, slot = step % 3
     float lvL[NL][3], lvR[NL][3];
     const int y0 = a - K;
+    // stores: lanes 1..62 whose vector starts inside the grid write all VD
+    // cells (a ring-column cell gets the ring value it holds, a cell past W
+    // the row's padding, which stays inside the row: the pitch is a multiple
+    // of VD cells >= W); every step issues one store, out of range before the
+    // last sweep's first output row
+    const __amdgpu_buffer_rsrc_t rd = plane_rsrc(s.dst, bytes);
+    const bool st_lane = lane >= 1 && lane <= kWave - 2 && xs < W;
+    // prologue: each load followed by a discarded store, the order of the
+    // steady state, so the wait counts at the loop head hold on every path
+    const u32x4_t none = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int q = 0; q < PD; ++q) {
         m2d_load<VD>(rs, roff(y0 + q), in[q]);
         oe[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, (int)eoff(y0 + q), 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b128(none, rd, (int)kNoAccess, 0, 0);
     }
-    float* __restrict__ dst = (float*)s.dst;
-    const bool out_lane = lane >= 1 && lane <= kWave - 2;
-    const bool full = out_lane && xs >= 1 && xs + VD <= W - 1;
 
-    // a computed row y of sweep l (1 <= l < K) in slot `sl`, patched where it
-    // holds ring cells; `prev2` is sweep l-2's row y (or the input row, l = 2)
+    // a computed row y of sweep l (1 <= l <= K), patched where it holds ring
+    // cells; `prev2` is sweep l-2's row y (or the input row, l = 2)
     auto patch = [&](int l, int y, float (&v)[VD], const float (&prev2)[VD], float oth_edge) {
         if (y == 0 || y == H - 1) {  // uniform
             const bool odd = l & 1;
@@ -245,20 +256,21 @@ __global__ __launch_bounds__(kMarch2DWaves * kWave) void k2d_march(Sweep2D s, Po
                     lvR[l - 1][s0] = from_upper_lane(o[0], 0.f);
                 }
             }
-            if (i >= 2 * K) {  // sweep K on row yi - K, stored
+            // sweep K on row yi - K (an interior row from step 2K on), stored
+            u32x4_t ov = none;
+            if (i >= 2 * K) {
                 float o[VD];
                 constexpr int m = NL - 1;
                 m2d_row<PAT, VD>(p, lv[m][s2], lvL[m][s2], lvR[m][s2], lv[m][s1], lvL[m][s1], lvR[m][s1], lv[m][s0],
                                  lvL[m][s0], lvR[m][s0], o);
-                float* row = dst + (size_t)(yi - K) * s.pitch + xs;
-                if (full) {
-                    *reinterpret_cast<float4*>(row) = make_float4(o[0], o[1], o[2], o[3]);
-                } else if (out_lane) {
+                // ring-column cells: sweep K-2's value (the input row, K = 2)
+                if (K == 2) patch(K, yi - K, o, in[i2], 0.f);
+                else patch(K, yi - K, o, lv[K >= 3 ? K - 3 : 0][s2], 0.f);
 #pragma unroll
-                    for (int j = 0; j < VD; ++j)
-                        if (xs + j >= 1 && xs + j < W - 1) row[j] = o[j];
-                }
+                for (int j = 0; j < VD; ++j) ov[j] = __float_as_uint(o[j]);
             }
+            const uint32_t so = (st_lane && i >= 2 * K) ? (uint32_t)(yi - K) * pb + (uint32_t)xs * 4u : kNoAccess;
+            __builtin_amdgcn_raw_buffer_store_b128(ov, rd, (int)so, 0, 0);
             return true;
         };
         if (!m2d_unroll(run, std::make_integer_sequence<int, NI>())) break;
