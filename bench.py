@@ -151,10 +151,12 @@ def tuner_pick(name):
     return out
 
 
-def tuner_regret(name, pick, sweeps=24, reps=3):
+def tuner_regret(name, pick, sweeps=None, reps=3):
     """Exhaustive search over the workload's candidates on its own grid
     (pskel_tuner_search, `sweeps` sweeps per timing, median of `reps`), and the
-    tuner's pick timed the same way: regret = pick / best - 1."""
+    tuner's pick timed the same way: regret = pick / best - 1.  By default 24
+    sweeps per timing, 240 on grids of at most 4M cells (whose sweeps take a
+    few microseconds: fewer would time mostly launch jitter)."""
     import ctypes as C
 
     from pskel_amd import _lib as L
@@ -162,6 +164,8 @@ def tuner_regret(name, pick, sweeps=24, reps=3):
     from pskel_amd import tuner as T
 
     w = WORKLOADS[name]
+    if sweeps is None:
+        sweeps = 240 if int(np.prod(w["shape"])) <= (1 << 22) else 24
     d = stencil_of(name).desc()
     with api.Grid(w["shape"], w["storage"], store_offset=store_offset_of(name), rounding=rounding_of(name),
                   seed=17) as g:
