@@ -246,8 +246,10 @@ pskel_status_t ensure_stage(pskel_grid_t* g)
 {
     const size_t rb = row_cells(g) * g->elem;
     const size_t want = std::max(rb, std::min(kStageBytes, rb * g->nrows));
-    if (g->stage_bytes >= want) return PSKEL_OK;
+    // no copy of an earlier transfer (one that failed part-way) still reads or
+    // writes the staging buffers
     PSKEL_HIP(hipStreamSynchronize(g->stream));
+    if (g->stage_bytes >= want) return PSKEL_OK;
     for (int b = 0; b < 2; ++b) {
         if (g->stage[b]) (void)hipHostFree(g->stage[b]);
         g->stage[b] = nullptr;
