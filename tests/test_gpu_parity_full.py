@@ -1,7 +1,8 @@
 """Parity at the full sizes of the benchmark workloads, against the oracle.
 
 * 9-point diffusion 4096^2 (the workload's grid): plain, tuned, and 4-sweep
-  temporal tiles of 16 rows, 12 sweeps, bit-exact.
+  temporal tiles of 16 rows, 12 sweeps, bit-exact; and the workload itself,
+  1000 sweeps through the register march and the LDS tiles, bit-exact.
 * Jacobi-3D 512^3 with bf16 and f16 storage, stochastic rounding, the fused
   pair (two sweeps per launch), 6 sweeps, bit-exact against the oracle's
   16-bit runner from the same initial words, FP64 residuals to 1e-12.
@@ -28,6 +29,20 @@ def test_diffusion9_4096(gpu):
         with api.Grid(a.shape, "f32") as g:
             g.upload_pair(a)
             g.run(s, 12, **kw)
+            got = g.download()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), kw
+
+
+def test_diffusion9_4096_x1000(gpu):
+    """The 9-point workload as configured (4096^2, 32 blobs, 1000 sweeps):
+    4 sweeps per launch in registers (the tuner's launch) and in LDS tiles."""
+    s = K.diffusion9()
+    a = inputs.blobs_field(4096, seed=3)
+    want = oracle.run_2d(s, a, 1000)
+    for kw in ({"time_tile": 4, "tile": (4, 0, 0)}, {"time_tile": 4, "tile": (0, 16, 0)}):
+        with api.Grid(a.shape, "f32") as g:
+            g.upload_pair(a)
+            g.run(s, 1000, variant="tuned", **kw)
             got = g.download()
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), kw
 
