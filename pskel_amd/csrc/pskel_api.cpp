@@ -611,6 +611,8 @@ pskel_status_t plan_run(const pskel_grid_t* g, const pskel_stencil_desc_t* d, co
                     return fail(PSKEL_ERR_UNSUPPORTED, "2D register march: tile (4, 0, rows per segment >= 0)");
                 if (!march2d_ok((int)g->W, (int)g->H, g->pitch))
                     return fail(PSKEL_ERR_UNSUPPORTED, "2D register march: grids wider than 8 cells, under 2 GiB");
+                // its ring handling is that of mask range 1 (rows 0 and H-1, columns 0 and W-1)
+                if (radius != 1) return fail(PSKEL_ERR_UNSUPPORTED, "2D register march: mask range 1");
             }
         }
         if (cfg.time_tile > 3 && ndim == 3) return fail(PSKEL_ERR_UNSUPPORTED, "3D temporal depth above 3");

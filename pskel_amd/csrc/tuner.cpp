@@ -114,8 +114,8 @@ std::vector<pskel_launch_cfg_t> candidates(const pskel_tune_query_t& q, const ps
         } else if (pat != PAT_CLOUD) {  // field stencils run one sweep per launch
             for (int k = 2; k <= 4; ++k)
                 for (int cy : {8, 16, 32, 64}) v.push_back(tuned(0, cy, 0, k));
-            // the register march: rows per segment (0: about 3 waves per SIMD)
-            if (march2d_fits_query(q))
+            // the register march (mask range 1): rows per segment (0: about 3 waves per SIMD)
+            if (march2d_fits_query(q) && desc_radius(d, 2) == 1)
                 for (int k = 2; k <= 4; ++k)
                     for (int seg : {0, 8, 24, 64}) v.push_back(tuned(4, 0, seg, k));
         }
@@ -237,7 +237,7 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
         // cells, 8 rows and 3 sweeps under it (the march there: 1.5-2x slower)
         const double cells = (double)q.width * q.height;
         if (q.iterations < 2) return tuned(0, 2, 4, 1);
-        if (cells >= 3.0 * (1 << 20) && march2d_fits_query(q))
+        if (cells >= 3.0 * (1 << 20) && march2d_fits_query(q) && desc_radius(d, 2) == 1)
             return tuned(4, 0, 0, (int)std::min<size_t>(q.iterations, 4));
         if (cells >= (double)(1 << 20)) return tuned(0, 16, 0, (int)std::min<size_t>(q.iterations, 4));
         return tuned(0, 8, 0, (int)std::min<size_t>(q.iterations, 3));

@@ -99,6 +99,13 @@ def test_register_march_refused(gpu):
         assert e.value.status == L.ERR_UNSUPPORTED
     with pytest.raises(L.PSkelError):
         _run(s, inputs.jacobi2d_field(40, 8, seed=1), 4, time_tile=4, tile=(4, 0, 0))
+    # a mask range above 1 widens the ring the march does not keep
+    s.radius = 3
+    with pytest.raises(L.PSkelError) as e:
+        _run(s, a, 4, time_tile=4, tile=(4, 0, 0))
+    assert e.value.status == L.ERR_UNSUPPORTED
+    got, _ = _run(s, a, 4, time_tile=4)  # the LDS tiles keep any range
+    assert _bits_equal(got, oracle.run_2d(s, a, 4))
 
 
 def test_register_march_full_size(gpu):

@@ -1,4 +1,6 @@
 """Tuner host logic that needs no device: candidate spaces, features, heuristic."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -19,6 +21,18 @@ def test_2d_march_candidates():
         assert sorted((c.time_tile, c.tile_z) for c in m) == [(k, s) for k in (2, 3, 4) for s in (0, 8, 24, 64)]
         assert march(_q((300, 8), "f32", name)) == []
     assert march(_q((300, 700), "u8", "life")) == []
+    # a mask range above 1: no march (it keeps a ring of one cell)
+    q = _q((3000, 3000), "f32", "jacobi2d", 100)
+    s = q.stencil()
+    s.radius = 3
+    cq, d = q.c_query(), s.desc()
+    buf = (L.LaunchCfg * L.TUNER_MAX_CANDIDATES)()
+    n = C.c_size_t()
+    L.check(L.lib().pskel_tuner_candidates(C.byref(cq), C.byref(d), buf, L.TUNER_MAX_CANDIDATES, C.byref(n)))
+    assert not any(buf[i].tile_x == 4 and buf[i].time_tile >= 2 for i in range(n.value))
+    h = L.LaunchCfg()
+    L.check(L.lib().pskel_tuner_heuristic(C.byref(cq), C.byref(d), C.byref(h)))
+    assert h.tile_x == 0 and h.time_tile == 4
 
 
 @pytest.mark.parametrize("q,expect_tuned", [
