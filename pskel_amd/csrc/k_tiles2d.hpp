@@ -279,7 +279,8 @@ __device__ __forceinline__ void ttile_body(const Sweep2D& s, const PT& p, int bx
     // the mask order.
     constexpr int NS = NT / 128;  // strips per column
     static_assert(LX <= 128, "one thread per column of the loaded region");
-    const int col = tid % 128, strip = tid / 128;
+    // a strip is two whole waves, so its row range is uniform across the wave
+    const int col = tid % 128, strip = __builtin_amdgcn_readfirstlane(tid / 128);
 #pragma unroll
     for (int j = 1; j <= K; ++j) {
         if (j > kgen) break;  // uniform
@@ -289,15 +290,17 @@ __device__ __forceinline__ void ttile_body(const Sweep2D& s, const PT& p, int bx
         const int ylo = j, yhi = LY - j;
         const int nx = xhi - xlo, ny = yhi - ylo;
         const int rows = (ny + NS - 1) / NS;
-        if (col < nx) {
-            const int x = xlo + col;
-            const int gw = gx0 + x;
-            const bool col_int = gw >= r && gw < Wd - r;
-            const int ya = ylo + strip * rows, yb = min(ya + rows, yhi);
+        const int x = xlo + col;
+        const int gw = gx0 + x;
+        // only interior cells are written: a ring column's lane and the ring
+        // rows of the strip compute nothing (their LDS cells keep the ring)
+        if (col < nx && gw >= r && gw < Wd - r) {
+            const int ys = ylo + strip * rows;
+            const int ya = max(ys, r - gy0), yb = min(min(ys + rows, yhi), H - r - gy0);
             const float* c = P + (ya - 1) * LX + x;
             float um = c[-1], u0 = c[0], up = c[1];                 // row y - 1
             float cm = c[LX - 1], c0 = c[LX], cp = c[LX + 1];       // row y
-#pragma unroll 2
+#pragma unroll 3
             for (int y = ya; y < yb; ++y) {
                 c += LX;
                 const float dm = c[LX - 1], d0 = c[LX], dp = c[LX + 1];  // row y + 1
@@ -318,8 +321,7 @@ __device__ __forceinline__ void ttile_body(const Sweep2D& s, const PT& p, int bx
                     acc = acc + p.wt[7] * d0;
                     acc = acc + p.wt[8] * dp;
                 }
-                const int gh = gy0 + y;
-                if (col_int && gh >= r && gh < H - r) Q[y * LX + x] = acc;
+                Q[y * LX + x] = acc;
                 um = cm; u0 = c0; up = cp;
                 cm = dm; c0 = d0; cp = dp;
             }
