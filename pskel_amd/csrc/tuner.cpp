@@ -231,16 +231,18 @@ pskel_launch_cfg_t heuristic(const pskel_tune_query_t& q, const pskel_stencil_de
         return tuned(0, 1, 4, 1);
     }
     if (q.ndim == 2 && (pat == PAT_J5 || pat == PAT_BOX9)) {
-        // from 3M cells the register march (4 sweeps, automatic segments):
-        // 1.2-1.8x the LDS tiles on every such grid measured; below, the LDS
-        // tiles, which keep more waves busy: 16 rows and 4 sweeps from 1M
-        // cells, 8 rows and 3 sweeps under it (the march there: 1.5-2x slower)
+        // From the tuner's timing table: the register march (4 sweeps,
+        // automatic segments) from 0.9M cells for the 5-point stencil and 4M
+        // for the 9-point one, which the LDS tiles hold out to longer (they
+        // issue its 17 products and sums with less overhead per cell); below,
+        // LDS tiles of 16 rows from 0.8M cells and of 8 rows under that, all
+        // at 4 sweeps (or the sweep count when smaller)
         const double cells = (double)q.width * q.height;
         if (q.iterations < 2) return tuned(0, 2, 4, 1);
-        if (cells >= 3.0 * (1 << 20) && march2d_fits_query(q) && desc_radius(d, 2) == 1)
-            return tuned(4, 0, 0, (int)std::min<size_t>(q.iterations, 4));
-        if (cells >= (double)(1 << 20)) return tuned(0, 16, 0, (int)std::min<size_t>(q.iterations, 4));
-        return tuned(0, 8, 0, (int)std::min<size_t>(q.iterations, 3));
+        const int k = (int)std::min<size_t>(q.iterations, 4);
+        const double march_from = pat == PAT_J5 ? 0.9e6 : 4.0e6;
+        if (cells >= march_from && march2d_fits_query(q) && desc_radius(d, 2) == 1) return tuned(4, 0, 0, k);
+        return tuned(0, cells >= 0.8e6 ? 16 : 8, 0, k);
     }
     if (q.ndim == 3 && pat == PAT_J7) {
         // the fused pair whose tiles the grid fills best (512-, 256- or

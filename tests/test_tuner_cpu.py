@@ -108,17 +108,21 @@ def test_3d_heuristic_fits_the_depth():
 
 
 def test_2d_heuristic():
-    """Jacobi and 9-point stencils: the register march from 3M cells, LDS
-    temporal tiles of 16 rows from 1M cells and of 8 rows (3 sweeps) below
-    (depth 4, or the sweep count when smaller); life: the plain kernel on small
-    grids, bit tiles on wide ones."""
+    """Jacobi and 9-point stencils: the register march from 0.9M cells (5-point)
+    or 4M cells (9-point), else LDS temporal tiles of 16 rows from 0.8M cells
+    and of 8 rows below, at depth 4 (or the sweep count when smaller); life:
+    the plain kernel on small grids, bit tiles on wide ones."""
     key = lambda c: (c.tile_x, c.tile_y, c.tile_z, c.time_tile)  # noqa: E731
-    assert key(T.heuristic_choice(_q((125, 205), "f32", "jacobi2d", 158))) == (0, 8, 0, 3)
-    assert key(T.heuristic_choice(_q((1024, 1331), "f32", "diffusion9", 29))) == (0, 16, 0, 4)
-    assert key(T.heuristic_choice(_q((1310, 2751), "f32", "diffusion9", 324))) == (4, 0, 0, 4)
-    assert key(T.heuristic_choice(_q((2903, 1983), "f32", "diffusion9", 3))) == (4, 0, 0, 3)
-    assert key(T.heuristic_choice(_q((2048, 2048), "f32", "jacobi2d", 2))) == (4, 0, 0, 2)
-    assert key(T.heuristic_choice(_q((300, 700), "f32", "jacobi2d", 2))) == (0, 8, 0, 2)
+    h = lambda shape, name, it: key(T.heuristic_choice(_q(shape, "f32", name, it)))  # noqa: E731
+    assert h((125, 205), "jacobi2d", 158) == (0, 8, 0, 4)
+    assert h((2226, 389), "jacobi2d", 24) == (0, 16, 0, 4)
+    assert h((1006, 979), "jacobi2d", 219) == (4, 0, 0, 4)
+    assert h((2048, 2048), "jacobi2d", 2) == (4, 0, 0, 2)
+    assert h((1024, 1331), "diffusion9", 29) == (0, 16, 0, 4)
+    assert h((2673, 908), "diffusion9", 58) == (0, 16, 0, 4)
+    assert h((4203, 1196), "diffusion9", 18) == (4, 0, 0, 4)
+    assert h((2903, 1983), "diffusion9", 3) == (4, 0, 0, 3)
+    assert h((300, 700), "diffusion9", 2) == (0, 8, 0, 2)
     assert T.heuristic_choice(_q((229, 71), "u8", "life", 107)).variant == 0
     assert T.heuristic_choice(_q((8194, 8194), "u8", "life", 500)).time_tile == 8
     assert T.heuristic_choice(_q((1683, 1653), "u8", "life", 29)).time_tile == 1
