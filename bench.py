@@ -424,6 +424,45 @@ def batch_choices(shapes, sweeps):
     return out
 
 
+def batch_choices_at_depth(shapes, sweeps, life_depth, f32_depth):
+    """Per grid: the model's best batch shape among those of the given depth
+    for its kind (None: the model's pick over every depth)."""
+    from pskel_amd import tuner as T
+
+    model = T.default_model()
+    out = []
+    for h, w, kind in shapes:
+        want = life_depth if kind == "life" else f32_depth
+        cands = [c for c in batch_candidates(kind) if want is None or max(1, int(c.time_tile)) == want]
+        pred = model.predict(T.features(batch_query(h, w, kind, sweeps), cands))
+        out.append(cands[int(np.argmin(pred))])
+    return out
+
+
+def batch_depth_search(shapes, timed, sweeps):
+    """The family depths measured on the batch itself.  The cost model prices a
+    grid alone, where a small grid's sweep is one launch of a few microseconds
+    whatever its kernel; inside the batch's persistent launches that floor is
+    shared, and only the work counts.  So each family's depth (life: 1, 3, 5 or
+    7 generations; f32: 1 or 3 sweeps) is timed on the whole batch, with every
+    grid's tile at that depth from the model, next to the model's own picks.
+    Returns (picks, table of {life_depth, f32_depth, ms_per_step})."""
+    table, best = [], None
+    for ld in (None, 1, 3, 5, 7):
+        for fd in (None, 1, 3):
+            if (ld is None) != (fd is None):
+                continue  # the model's picks, or a depth for both families
+            picks = batch_choices_at_depth(shapes, sweeps, ld, fd)
+            rows = [int(c.tile_y) for c in picks]
+            depths = [max(1, int(c.time_tile)) for c in picks]
+            ms = timed(rows, depths, 2, 1)[0] * 1e3 / 2
+            table.append({"life_depth": ld if ld is not None else "model", "f32_depth": fd if fd is not None else "model",
+                          "ms_per_step": round(ms, 3)})
+            if best is None or ms < best[0]:
+                best = (ms, picks)
+    return best[1], table
+
+
 def batch_regret_sample(shapes, picks, sweeps, nsample=16, reps=3):
     """Regret of the model's batch choices on a sample of the grids: each
     sampled grid alone, every batch-supported shape timed (pskel_tuner_measure)."""
@@ -457,10 +496,11 @@ def batch_regret_sample(shapes, picks, sweeps, nsample=16, reps=3):
 
 def measure_batch(steps, warmup, sweeps=200, compare=True):
     """256 uneven grids, `sweeps` sweeps per step: persistent kernels per
-    launch, captured in one hipGraph, with the launch shape and temporal depth
-    of every grid from the cost model; beside it the same batch at depth 1
-    (register tiles, the model's rows per lane) and the same work launched
-    grid by grid (one runIterativeGPU per grid, the reference's way)."""
+    launch, captured in one hipGraph, with every grid's launch shape from the
+    cost model at the family depths measured on the batch (batch_depth_search);
+    beside it the model's own picks, the same batch at depth 1 (register
+    tiles, the model's rows per lane) and the same work launched grid by grid
+    (one runIterativeGPU per grid, the reference's way)."""
     from pskel_amd import api
     from pskel_amd import kernels as K
     from pskel_amd import tuner as T
@@ -472,12 +512,36 @@ def measure_batch(steps, warmup, sweeps=200, compare=True):
         g.upload_pair(f)
         grids.append(g)
         stencils.append(K.life() if kind == "life" else K.jacobi2d())
-    picks = batch_choices(shapes, sweeps)
-    how = "cost model" if picks is not None else "default (depth 1, 4 rows per lane)"
-    if picks is None:
+    cells = sum(h * w for h, w, _ in shapes)
+    nbytes = sum(h * w * (1 if k == "life" else 4) * 2 for h, w, k in shapes)
+
+    def timed(rows_, depths_, steps_=steps, warmup_=warmup):
+        # the grids restart from the same fields, so every configuration times the same work
+        for g, f in zip(grids, fields):
+            g.upload_pair(f)
+        with api.Batch(grids, stencils, rows_, depths_) as b:
+            for _ in range(warmup_):
+                b.run(sweeps, use_graph=True)
+            api.synchronize()
+            t0 = time.perf_counter()
+            dev = 0.0
+            launches = 0
+            for _ in range(steps_):
+                st = b.run(sweeps, use_graph=True)
+                dev += st["elapsed_ms"]
+                launches = st["launches"]
+            api.synchronize()
+            return time.perf_counter() - t0, dev, launches, b.info()
+
+    search = None
+    model_picks = batch_choices(shapes, sweeps)
+    if model_picks is None:
+        how = "default (depth 1, 4 rows per lane)"
         rows, depths = [4] * len(shapes), [1] * len(shapes)
         rows1 = rows
     else:
+        how = "cost model tiles at the family depths measured on the batch"
+        picks, search = batch_depth_search(shapes, timed, sweeps)
         rows = [int(c.tile_y) for c in picks]
         depths = [max(1, int(c.time_tile)) for c in picks]
         # depth 1 beside it: the model's best register tile per grid
@@ -487,23 +551,6 @@ def measure_batch(steps, warmup, sweeps=200, compare=True):
             cands = batch_candidates(kind)[:4]
             pred = model.predict(T.features(batch_query(h, w, kind, sweeps), cands))
             rows1.append(int(cands[int(np.argmin(pred))].tile_y))
-    cells = sum(h * w for h, w, _ in shapes)
-    nbytes = sum(h * w * (1 if k == "life" else 4) * 2 for h, w, k in shapes)
-
-    def timed(rows_, depths_):
-        with api.Batch(grids, stencils, rows_, depths_) as b:
-            for _ in range(warmup):
-                b.run(sweeps, use_graph=True)
-            api.synchronize()
-            t0 = time.perf_counter()
-            dev = 0.0
-            launches = 0
-            for _ in range(steps):
-                st = b.run(sweeps, use_graph=True)
-                dev += st["elapsed_ms"]
-                launches = st["launches"]
-            api.synchronize()
-            return time.perf_counter() - t0, dev, launches, b.info()
 
     try:
         wall, dev, launches, info = timed(rows, depths)
@@ -531,13 +578,21 @@ def measure_batch(steps, warmup, sweeps=200, compare=True):
         "launch_shapes_from": how,
         "cells": cells,
     }
+    if search is not None:
+        out["depth_search"] = search
+        model_ms = next(r["ms_per_step"] for r in search if r["life_depth"] == "model")
+        out["model_picks_ms_per_step"] = model_ms
+        out["speedup_vs_model_picks"] = model_ms / (wall * 1e3 / steps)
+        # the model's picks against the best of the batch-level search
+        out["model_regret_vs_batch_search"] = model_ms / min(r["ms_per_step"] for r in search) - 1.0
     if not compare:
         return out
     out.update({"depth1_ms_per_step": wall1 * 1e3 / steps, "depth1_launch_steps": launches1,
                 "speedup_vs_depth1": wall1 / wall, "separate_launches_ms": sep * 1e3,
                 "speedup_vs_separate": sep / (wall / steps)})
-    if picks is not None:
-        reg = batch_regret_sample(shapes, picks, sweeps)
+    if model_picks is not None:
+        # the model's per-grid picks against each grid's exhaustive best, each grid timed alone
+        reg = batch_regret_sample(shapes, model_picks, sweeps)
         out["regret_sample"] = {"grids": len(reg), "mean": float(np.mean([r["regret"] for r in reg])),
                                 "max": float(np.max([r["regret"] for r in reg])), "per_grid": reg}
     return out
@@ -795,8 +850,10 @@ def main(argv=None):
         extra["batch256:graph"] = rb
         print(f"# batch256: {rb['cell_updates_per_s']:.4g} CU/s, {rb['ms_per_step']:.2f} ms/step, "
               f"{rb['speedup_vs_depth1']:.2f}x vs depth 1, {rb['speedup_vs_separate']:.1f}x vs separate launches, "
-              f"depths {rb['depths']}, sample regret "
-              f"{rb.get('regret_sample', {}).get('mean', float('nan')):.3f}", file=sys.stderr, flush=True)
+              f"depths {rb['depths']}, model's own picks {rb.get('model_picks_ms_per_step', float('nan')):.2f} ms/step "
+              f"(regret {rb.get('model_regret_vs_batch_search', float('nan')):.3f} vs the batch search), "
+              f"per-grid sample regret {rb.get('regret_sample', {}).get('mean', float('nan')):.3f}",
+              file=sys.stderr, flush=True)
         rd = measure_dropin()
         extra["jacobi3d:dropin"] = rd
         if rd and "gpu_ms" in rd:
