@@ -494,7 +494,7 @@ def batch_regret_sample(shapes, picks, sweeps, nsample=16, reps=3):
     return rows
 
 
-def measure_batch(steps, warmup, sweeps=200, compare=True):
+def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
     """256 uneven grids, `sweeps` sweeps per step: persistent kernels per
     launch, captured in one hipGraph, with every grid's launch shape from the
     cost model at the family depths measured on the batch (batch_depth_search);
@@ -540,8 +540,12 @@ def measure_batch(steps, warmup, sweeps=200, compare=True):
         rows, depths = [4] * len(shapes), [1] * len(shapes)
         rows1 = rows
     else:
-        how = "cost model tiles at the family depths measured on the batch"
-        picks, search = batch_depth_search(shapes, timed, sweeps)
+        if family_depths is None:
+            how = "cost model tiles at the family depths measured on the batch"
+            picks, search = batch_depth_search(shapes, timed, sweeps)
+        else:
+            how = f"cost model tiles at life depth {family_depths[0]}, f32 depth {family_depths[1]}"
+            picks = batch_choices_at_depth(shapes, sweeps, *family_depths)
         rows = [int(c.tile_y) for c in picks]
         depths = [max(1, int(c.time_tile)) for c in picks]
         # depth 1 beside it: the model's best register tile per grid
@@ -785,6 +789,8 @@ def main(argv=None):
                     help="only run the plain copy kernel on two buffers of this size (PMC calibration)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the 256-grid batch with the model's launch shapes (no comparisons)")
+    ap.add_argument("--batch-depths", default="",
+                    help="with --batch-only: life,f32 family depths (no depth search; a profiling target)")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -802,8 +808,9 @@ def main(argv=None):
     if args.copy_mib:
         return copy_calibration(args.copy_mib, max(args.steps, 1))
     if args.batch_only:
-        # the batch alone, with the model's depths (a profiling target)
-        print(json.dumps(measure_batch(max(args.steps, 1), args.warmup, compare=False)))
+        # the batch alone (a profiling target): the depth search, or the given family depths
+        depths = tuple(int(x) for x in args.batch_depths.split(",")) if args.batch_depths else None
+        print(json.dumps(measure_batch(max(args.steps, 1), args.warmup, compare=False, family_depths=depths)))
         return 0
 
     run_kw = None
