@@ -384,7 +384,18 @@ def batch_fields(seed=5):
 
 # batch-supported launch shapes per grid kind: depth 1 register tiles (rows per
 # lane), f32 LDS temporal tiles of 3 sweeps (16 or 32 rows), bit-packed life
-# tiles of 3, 5 or 7 generations
+# tiles of 3, 5 or 7 generations.  The f32 register march of 3 sweeps is a
+# family-wide choice of the batch search (MARCH_F32), not a per-grid candidate.
+MARCH_F32 = "m"
+MARCH_SEG = 32  # rows per march segment (16, 32 and 64 measured on the batch: 32 fastest)
+
+
+def batch_march(shapes, f32_depth):
+    """Per grid: the march's rows per segment (0: not marched) when the f32
+    family runs the register march, every f32 grid over 8 cells wide."""
+    return [MARCH_SEG if f32_depth == MARCH_F32 and kind != "life" and w > 8 else 0 for h, w, kind in shapes]
+
+
 def batch_candidates(kind):
     from pskel_amd import _lib as L
 
@@ -426,13 +437,14 @@ def batch_choices(shapes, sweeps):
 
 def batch_choices_at_depth(shapes, sweeps, life_depth, f32_depth):
     """Per grid: the model's best batch shape among those of the given depth
-    for its kind (None: the model's pick over every depth)."""
+    for its kind (None: the model's pick over every depth; MARCH_F32: depth 3,
+    whose LDS tile stays the shape of a grid the march cannot take)."""
     from pskel_amd import tuner as T
 
     model = T.default_model()
     out = []
     for h, w, kind in shapes:
-        want = life_depth if kind == "life" else f32_depth
+        want = life_depth if kind == "life" else (3 if f32_depth == MARCH_F32 else f32_depth)
         cands = [c for c in batch_candidates(kind) if want is None or max(1, int(c.time_tile)) == want]
         pred = model.predict(T.features(batch_query(h, w, kind, sweeps), cands))
         out.append(cands[int(np.argmin(pred))])
@@ -444,23 +456,27 @@ def batch_depth_search(shapes, timed, sweeps):
     grid alone, where a small grid's sweep is one launch of a few microseconds
     whatever its kernel; inside the batch's persistent launches that floor is
     shared, and only the work counts.  So each family's depth (life: 1, 3, 5 or
-    7 generations; f32: 1 or 3 sweeps) is timed on the whole batch, with every
-    grid's tile at that depth from the model, next to the model's own picks.
-    Returns (picks, table of {life_depth, f32_depth, ms_per_step})."""
+    7 generations; f32: 1 or 3 sweeps in LDS tiles, or 3 through the register
+    march, MARCH_F32) is timed on the whole batch, with every grid's tile at
+    that depth from the model, next to the model's own picks.
+    Returns (picks, march segments, f32 choice, table of {life_depth,
+    f32_depth, ms_per_step})."""
     table, best = [], None
     for ld in (None, 1, 3, 5, 7):
-        for fd in (None, 1, 3):
+        for fd in (None, 1, 3, MARCH_F32):
             if (ld is None) != (fd is None):
                 continue  # the model's picks, or a depth for both families
             picks = batch_choices_at_depth(shapes, sweeps, ld, fd)
             rows = [int(c.tile_y) for c in picks]
             depths = [max(1, int(c.time_tile)) for c in picks]
-            ms = timed(rows, depths, 2, 1)[0] * 1e3 / 2
-            table.append({"life_depth": ld if ld is not None else "model", "f32_depth": fd if fd is not None else "model",
+            march = batch_march(shapes, fd)
+            ms = timed(rows, depths, march, 2, 1)[0] * 1e3 / 2
+            table.append({"life_depth": ld if ld is not None else "model",
+                          "f32_depth": {None: "model", MARCH_F32: "3 (register march)"}.get(fd, fd),
                           "ms_per_step": round(ms, 3)})
             if best is None or ms < best[0]:
-                best = (ms, picks)
-    return best[1], table
+                best = (ms, picks, march, fd)
+    return best[1], best[2], best[3], table
 
 
 def batch_regret_sample(shapes, picks, sweeps, nsample=16, reps=3):
@@ -497,8 +513,8 @@ def batch_regret_sample(shapes, picks, sweeps, nsample=16, reps=3):
 def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
     """256 uneven grids, `sweeps` sweeps per step: persistent kernels per
     launch, captured in one hipGraph, with every grid's launch shape from the
-    cost model at the family depths measured on the batch (batch_depth_search);
-    beside it the model's own picks, the same batch at depth 1 (register
+    cost model at the family depths measured on the batch (batch_depth_search;
+    the f32 family in LDS tiles or through the register march); beside it the model's own picks, the same batch at depth 1 (register
     tiles, the model's rows per lane) and the same work launched grid by grid
     (one runIterativeGPU per grid, the reference's way)."""
     from pskel_amd import api
@@ -515,11 +531,11 @@ def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
     cells = sum(h * w for h, w, _ in shapes)
     nbytes = sum(h * w * (1 if k == "life" else 4) * 2 for h, w, k in shapes)
 
-    def timed(rows_, depths_, steps_=steps, warmup_=warmup):
+    def timed(rows_, depths_, march_=None, steps_=steps, warmup_=warmup):
         # the grids restart from the same fields, so every configuration times the same work
         for g, f in zip(grids, fields):
             g.upload_pair(f)
-        with api.Batch(grids, stencils, rows_, depths_) as b:
+        with api.Batch(grids, stencils, rows_, depths_, march_) as b:
             for _ in range(warmup_):
                 b.run(sweeps, use_graph=True)
             api.synchronize()
@@ -534,6 +550,7 @@ def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
             return time.perf_counter() - t0, dev, launches, b.info()
 
     search = None
+    march, f32_choice = None, None
     model_picks = batch_choices(shapes, sweeps)
     if model_picks is None:
         how = "default (depth 1, 4 rows per lane)"
@@ -542,10 +559,12 @@ def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
     else:
         if family_depths is None:
             how = "cost model tiles at the family depths measured on the batch"
-            picks, search = batch_depth_search(shapes, timed, sweeps)
+            picks, march, f32_choice, search = batch_depth_search(shapes, timed, sweeps)
         else:
             how = f"cost model tiles at life depth {family_depths[0]}, f32 depth {family_depths[1]}"
+            f32_choice = family_depths[1]
             picks = batch_choices_at_depth(shapes, sweeps, *family_depths)
+            march = batch_march(shapes, f32_choice)
         rows = [int(c.tile_y) for c in picks]
         depths = [max(1, int(c.time_tile)) for c in picks]
         # depth 1 beside it: the model's best register tile per grid
@@ -557,7 +576,7 @@ def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
             rows1.append(int(cands[int(np.argmin(pred))].tile_y))
 
     try:
-        wall, dev, launches, info = timed(rows, depths)
+        wall, dev, launches, info = timed(rows, depths, march)
         if compare:
             wall1, _, launches1, _ = timed(rows1, [1] * len(shapes))
             # separate launches: every grid on its own, tuned kernel, no graph
@@ -577,6 +596,9 @@ def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
         "launch_steps": launches,
         "hbm_roofline_frac_per_sweep_bytes": nbytes * sweeps * steps / wall / (HBM_PEAK_GBS * 1e9),
         "depths": {str(k): depths.count(k) for k in sorted(set(depths))},
+        "f32_family": ("register march, 3 sweeps per launch, %d-row segments" % MARCH_SEG
+                       if f32_choice == MARCH_F32 else "LDS temporal tiles" if 3 in depths else "register tiles"),
+        "march_grids": sum(1 for m in (march or []) if m),
         "items": info["items"],
         "persistent_blocks": info["blocks"],
         "launch_shapes_from": how,
@@ -790,7 +812,8 @@ def main(argv=None):
     ap.add_argument("--batch-only", action="store_true",
                     help="only the 256-grid batch with the model's launch shapes (no comparisons)")
     ap.add_argument("--batch-depths", default="",
-                    help="with --batch-only: life,f32 family depths (no depth search; a profiling target)")
+                    help="with --batch-only: life,f32 family depths, f32 'm' for the register march "
+                         "(no depth search; a profiling target)")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -809,7 +832,10 @@ def main(argv=None):
         return copy_calibration(args.copy_mib, max(args.steps, 1))
     if args.batch_only:
         # the batch alone (a profiling target): the depth search, or the given family depths
-        depths = tuple(int(x) for x in args.batch_depths.split(",")) if args.batch_depths else None
+        depths = None
+        if args.batch_depths:
+            life_d, f32_d = args.batch_depths.split(",")
+            depths = (int(life_d), f32_d if f32_d == MARCH_F32 else int(f32_d))
         print(json.dumps(measure_batch(max(args.steps, 1), args.warmup, compare=False, family_depths=depths)))
         return 0
 
@@ -857,7 +883,7 @@ def main(argv=None):
         extra["batch256:graph"] = rb
         print(f"# batch256: {rb['cell_updates_per_s']:.4g} CU/s, {rb['ms_per_step']:.2f} ms/step, "
               f"{rb['speedup_vs_depth1']:.2f}x vs depth 1, {rb['speedup_vs_separate']:.1f}x vs separate launches, "
-              f"depths {rb['depths']}, model's own picks {rb.get('model_picks_ms_per_step', float('nan')):.2f} ms/step "
+              f"depths {rb['depths']}, f32 {rb['f32_family']}, model's own picks {rb.get('model_picks_ms_per_step', float('nan')):.2f} ms/step "
               f"(regret {rb.get('model_regret_vs_batch_search', float('nan')):.3f} vs the batch search), "
               f"per-grid sample regret {rb.get('regret_sample', {}).get('mean', float('nan')):.3f}",
               file=sys.stderr, flush=True)

@@ -344,7 +344,11 @@ pskel_status_t pskel_grid_set_field(pskel_grid_t* grid, int32_t slot, pskel_fiel
  *      grid's state to its other buffer.  N sweeps run as N / k launches of k
  *      and the rest as one odd launch or two (r - 1, then 1).  cfgs[i].tile_y:
  *      at depth 1 the rows per lane (1, 2, 4, 8; default 4), at depth 3 in F32
- *      the tile rows (16 or 32; default 16).  cfgs may be NULL (depth 1, 4 rows). */
+ *      the tile rows (16 or 32; default 16).  F32 at depth 3 with tile_x = 4
+ *      runs the register march instead (mask range 1, grids over 8 cells wide):
+ *      one wave per tile of 62 x 4 columns and segment of tile_z rows (0: 32),
+ *      its single-sweep tail launches through register tiles.  cfgs may be
+ *      NULL (depth 1, 4 rows). */
 typedef struct pskel_batch pskel_batch_t;
 pskel_status_t pskel_batch_create(pskel_grid_t* const* grids, const pskel_stencil_desc_t* descs,
                                   const pskel_launch_cfg_t* cfgs, size_t n, pskel_batch_t** out);

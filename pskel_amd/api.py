@@ -211,9 +211,11 @@ class Batch:
     (pskel_batch_t).  grids: list of Grid; stencils: one per grid; rows: rows
     per lane per grid (1, 2, 4 or 8) at depth 1, tile rows at depth > 1 (f32:
     16 or 32; life: 32); depths: sweeps fused per launch per grid (odd: 1 or 3
-    for f32, 1, 3, 5 or 7 for life)."""
+    for f32, 1, 3, 5 or 7 for life); march: per grid, 0 or, for an f32 grid at
+    depth 3, the rows per segment of the register march that replaces its LDS
+    tiles (tile (4, 0, rows))."""
 
-    def __init__(self, grids, stencils, rows=None, depths=None):
+    def __init__(self, grids, stencils, rows=None, depths=None, march=None):
         n = len(grids)
         self.grids = list(grids)
         hs = (C.c_void_p * n)(*[g._h for g in grids])
@@ -224,6 +226,8 @@ class Batch:
             k = int(depths[i]) if depths is not None else 1
             cs[i].time_tile = k
             cs[i].tile_y = int(rows[i]) if rows is not None and rows[i] else (4 if k == 1 else 0)
+            if march is not None and march[i]:
+                cs[i].tile_x, cs[i].tile_y, cs[i].tile_z = 4, 0, int(march[i])
         h = C.c_void_p()
         L.check(L.lib().pskel_batch_create(hs, ds, cs, n, C.byref(h)))
         self._h = h

@@ -25,6 +25,8 @@
 #include <vector>
 
 #include "k_tiles2d.hpp"
+#define PSKEL_MARCH2D_INSTANTIATE
+#include "k_march2d.hpp"
 
 using namespace pskel;
 
@@ -39,7 +41,9 @@ struct BatchEntry {
     int pitch;
     int radius;
     int kind;
-    int ry;           // K = 1: rows per lane; K > 1: tile rows (f32: CY; life: bit-tile rows R)
+    int ry;           // K = 1: rows per lane; K > 1: tile rows (f32: CY; life: bit-tile rows R;
+                      // register march: rows per segment)
+    int march;        // f32 at K = 3 through the register march (one wave per tile and segment)
     int depth;        // K, sweeps fused per launch (odd)
     int full;         // launches of K sweeps
     int tail0, tail1; // sweeps of the launches after them (odd; 0: none)
@@ -99,9 +103,32 @@ __device__ __forceinline__ void run_life_bits(const BatchEntry& E, const Sweep2D
     }
 }
 
+// A register-march grid: launches of 3 sweeps run one wave per (tile,
+// segment) unit, four units per item; its single-sweep tail launches cover the
+// grid with register tiles of 4 rows per lane (256 columns x 16 rows), each
+// of the grid's items taking every items-th tile.
+__device__ __forceinline__ void run_march(const BatchEntry& E, const Sweep2D& s, int item_t, int k)
+{
+    const int lane = (int)threadIdx.x, wave = (int)threadIdx.y;
+    if (k == 3) {
+        const int g = item_t * kBatchWaves + wave;
+        if (g >= E.ntiles) return;  // the whole wave; the march does not synchronise
+        if (E.kind == BK_J5) march2d_wave<PAT_J5, 3, 3>(s, E, E.tiles_x, E.ry, g, lane);
+        else march2d_wave<PAT_BOX9, 3, 3>(s, E, E.tiles_x, E.ry, g, lane);
+        return;
+    }
+    const int ni = (E.ntiles + kBatchWaves - 1) / kBatchWaves;  // the grid's items
+    const int rtx = (E.W + 255) / 256, nrt = rtx * ((E.H + 4 * kBatchWaves - 1) / (4 * kBatchWaves));
+    for (int rt = item_t; rt < nrt; rt += ni) {
+        if (E.kind == BK_J5) tile_f32<PAT_J5, 4>(s, E, rt % rtx, rt / rtx);
+        else tile_f32<PAT_BOX9, 4>(s, E, rt % rtx, rt / rtx);
+    }
+}
+
 // Work families, one kernel each so each keeps the registers its body needs
-// (register and SWAR tiles, f32 LDS temporal tiles, bit-packed life tiles).
-enum BatchFamily : int { BF_REG = 0, BF_TTILE = 1, BF_BITS = 2, BF_COUNT = 3 };
+// (register and SWAR tiles, f32 LDS temporal tiles, bit-packed life tiles,
+// register-march units).
+enum BatchFamily : int { BF_REG = 0, BF_TTILE = 1, BF_BITS = 2, BF_MARCH = 3, BF_COUNT = 4 };
 
 template <int FAM>
 __global__ __launch_bounds__(256) void k_batch(const BatchEntry* __restrict__ tab, const int* __restrict__ item_grid,
@@ -132,6 +159,8 @@ __global__ __launch_bounds__(256) void k_batch(const BatchEntry* __restrict__ ta
             }
         } else if constexpr (FAM == BF_BITS) {
             run_life_bits(E, s, t, k);
+        } else if constexpr (FAM == BF_MARCH) {
+            run_march(E, s, t, k);
         } else {
             if (E.kind == BK_J5) run_ttile<PAT_J5>(E, s, t, k, lds, lds + kBatchLds);
             else run_ttile<PAT_BOX9>(E, s, t, k, lds, lds + kBatchLds);
@@ -139,7 +168,12 @@ __global__ __launch_bounds__(256) void k_batch(const BatchEntry* __restrict__ ta
     }
 }
 
-int family_of(const BatchEntry& e) { return e.depth == 1 ? BF_REG : (e.kind == BK_LIFE ? BF_BITS : BF_TTILE); }
+int family_of(const BatchEntry& e)
+{
+    if (e.depth == 1) return BF_REG;
+    if (e.kind == BK_LIFE) return BF_BITS;
+    return e.march ? BF_MARCH : BF_TTILE;
+}
 
 // the schedule of N sweeps at depth K: full launches of K, then the tail
 void schedule(BatchEntry& e, size_t n)
@@ -158,8 +192,12 @@ void schedule(BatchEntry& e, size_t n)
 int launches_of(const BatchEntry& e) { return e.full + (e.tail0 > 0) + (e.tail1 > 0); }
 
 // items of one grid: register and SWAR tiles and f32 temporal tiles are one
-// block each, bit-packed life tiles one wave each (four per item)
-int items_of(const BatchEntry& e) { return e.kind == BK_LIFE && e.depth > 1 ? (e.ntiles + 3) / 4 : e.ntiles; }
+// block each, bit-packed life tiles and register-march units one wave each
+// (four per item)
+int items_of(const BatchEntry& e)
+{
+    return (e.kind == BK_LIFE && e.depth > 1) || e.march ? (e.ntiles + kBatchWaves - 1) / kBatchWaves : e.ntiles;
+}
 
 }  // namespace
 
@@ -253,7 +291,8 @@ hipError_t issue_batch(pskel_batch_t* b, hipStream_t st)
             const dim3 grid((unsigned)std::min(n, b->fam_blocks[f]));
             if (f == BF_REG) hipLaunchKernelGGL(k_batch<BF_REG>, grid, block, 0, fs, b->d_tab, items, n, l);
             else if (f == BF_TTILE) hipLaunchKernelGGL(k_batch<BF_TTILE>, grid, block, 0, fs, b->d_tab, items, n, l);
-            else hipLaunchKernelGGL(k_batch<BF_BITS>, grid, block, 0, fs, b->d_tab, items, n, l);
+            else if (f == BF_BITS) hipLaunchKernelGGL(k_batch<BF_BITS>, grid, block, 0, fs, b->d_tab, items, n, l);
+            else hipLaunchKernelGGL(k_batch<BF_MARCH>, grid, block, 0, fs, b->d_tab, items, n, l);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipEventRecord(b->join[f], fs);
@@ -318,6 +357,18 @@ pskel_status_t pskel_batch_create(pskel_grid_t* const* grids, const pskel_stenci
             const int cr = kLifeRows - 2 * depth;
             e.tiles_x = std::max(1, (e.W - 1 + CW - 1) / CW);
             e.ntiles = e.tiles_x * std::max(1, (e.H - 1 + cr - 1) / cr);
+        } else if (cfgs && cfgs[i].tile_x == kMarch2DVd) {
+            // register march, 3 sweeps per launch: tiles of 62 lane vectors along w,
+            // segments of tile_z rows (0: 32)
+            if (depth != 3) return reject(PSKEL_ERR_UNSUPPORTED, "the batched register march runs 3 sweeps per launch");
+            if (radius != 1 || !march2d_fits(e.W, e.H, g->pitch))
+                return reject(PSKEL_ERR_UNSUPPORTED, "the batched register march needs mask range 1 and a grid over 8 cells wide");
+            const March2DShape m = march2d_shape(e.W, e.H, cfgs[i].tile_z > 0 ? cfgs[i].tile_z : 32);
+            if (m.units < 1) return reject(PSKEL_ERR_UNSUPPORTED, "the batched register march needs interior rows");
+            e.march = 1;
+            e.ry = m.seg;
+            e.tiles_x = m.ntw;
+            e.ntiles = m.units;
         } else {
             // LDS temporal tiles of 120 columns and 16 or 32 rows, 3 sweeps per launch
             if (depth != kTTileK) return reject(PSKEL_ERR_UNSUPPORTED, "batched f32 depths are 1 or 3");

@@ -103,9 +103,10 @@ __device__ __forceinline__ void m2d_load(const __amdgpu_buffer_rsrc_t& r, uint32
 }
 
 // one output row of a sweep: u, c, d are the rows above, at and below, with
-// their left (cell x0-1) and right (cell x0+VD) neighbours from the lanes beside
-template <int PAT, int VD>
-__device__ __forceinline__ void m2d_row(const Points& p, const float (&u)[VD], float uL, float uR,
+// their left (cell x0-1) and right (cell x0+VD) neighbours from the lanes beside;
+// p holds the scaled sum's coeff and the 3x3 weights wt
+template <int PAT, int VD, class PT>
+__device__ __forceinline__ void m2d_row(const PT& p, const float (&u)[VD], float uL, float uR,
                                         const float (&c)[VD], float cL, float cR, const float (&d)[VD], float dL,
                                         float dR, float (&o)[VD])
 {
@@ -135,18 +136,17 @@ __device__ __forceinline__ void m2d_row(const Points& p, const float (&u)[VD], f
     }
 }
 
-template <int PAT, int K, int PD>
-__global__ __launch_bounds__(kMarch2DWaves * kWave) void k2d_march(Sweep2D s, Points p, int ntw, int seg, int units)
+// One wave's unit g (tile g % ntw, segment g / ntw, g < ntw * segments):
+// K sweeps of the segment's rows.  Waves are independent (no barrier), so a
+// caller may run any set of units in any block.
+template <int PAT, int K, int PD, class PT>
+__device__ __forceinline__ void march2d_wave(const Sweep2D& s, const PT& p, int ntw, int seg, int g, int lane)
 {
     constexpr int VD = kMarch2DVd;
     constexpr int NL = K - 1;     // sweeps kept in rings (all but the last)
     constexpr int NI = PD + 3;    // input slots: rows yi-2 .. yi + PD
     static_assert(K >= 2 && K <= VD, "the halo lane vector covers K cells");
     static_assert(PD % 3 == 0, "the step loop unrolls over the input slots, a multiple of the three-row rings");
-    const int lane = threadIdx.x % kWave;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int g = (int)xcd_remap(blockIdx.x, gridDim.x) * kMarch2DWaves + wave;
-    if (g >= units) return;  // the whole wave
     const int t = g % ntw, sg = g / ntw;
     const int H = s.H, W = s.W;
     const int a = 1 + sg * seg, e = min(a + seg, H - 1);
@@ -276,6 +276,16 @@ __global__ __launch_bounds__(kMarch2DWaves * kWave) void k2d_march(Sweep2D s, Po
         };
         if (!m2d_unroll(run, std::make_integer_sequence<int, NI>())) break;
     }
+}
+
+template <int PAT, int K, int PD>
+__global__ __launch_bounds__(kMarch2DWaves * kWave) void k2d_march(Sweep2D s, Points p, int ntw, int seg, int units)
+{
+    const int lane = threadIdx.x % kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int g = (int)xcd_remap(blockIdx.x, gridDim.x) * kMarch2DWaves + wave;
+    if (g >= units) return;  // the whole wave
+    march2d_wave<PAT, K, PD>(s, p, ntw, seg, g, lane);
 }
 
 }  // namespace

@@ -76,6 +76,43 @@ def test_batch_temporal_depths(gpu, use_graph):
             g.close()
 
 
+MARCH_SHAPES = [(300, 700, "jacobi"), (129, 513, "diff9"), (9, 20, "jacobi"), (1000, 260, "diff9"),
+                (64, 64, "jacobi"), (70, 1030, "life"), (257, 66, "jacobi"), (40, 2100, "diff9")]
+MARCH_DEPTHS = [3, 3, 3, 3, 3, 7, 1, 3]
+MARCH_ROWS = [0, 0, 0, 0, 16, 0, 4, 0]
+MARCH_SEGS = [32, 8, 32, 100, 0, 0, 0, 5]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_register_march(gpu, use_graph):
+    """f32 grids through the register march (segments of 5 to 100 rows, tiny
+    and wide grids), beside LDS tiles, bit tiles and register tiles; runs of
+    13, 8, 1 and 14 sweeps take no, one or two single-sweep tail launches."""
+    grids, stencils, fields = _make(MARCH_SHAPES, seed=11)
+    try:
+        with api.Batch(grids, stencils, MARCH_ROWS, MARCH_DEPTHS, MARCH_SEGS) as b:
+            done = 0
+            for n in (13, 8, 1, 14):
+                b.run(n, use_graph=use_graph)
+                done += n
+                for g, s, f, m in zip(grids, stencils, fields, MARCH_SEGS):
+                    assert np.array_equal(g.download(), oracle.run_2d(s, f, done)), (g.shape, m, done)
+    finally:
+        for g in grids:
+            g.close()
+
+
+def test_batch_march_validation(gpu):
+    grids, stencils, _ = _make([(20, 8, "jacobi"), (64, 64, "jacobi")], seed=12)
+    try:
+        for depths, segs in (([3, 3], [32, 0]), ([1, 5], [0, 32])):
+            with pytest.raises(PSkelError):
+                api.Batch(grids, stencils, None, depths, segs)
+    finally:
+        for g in grids:
+            g.close()
+
+
 def test_batch_depth_validation(gpu):
     grids, stencils, _ = _make([(64, 64, "jacobi"), (64, 64, "life")], seed=9)
     try:
