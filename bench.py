@@ -382,101 +382,8 @@ def batch_fields(seed=5):
     return shapes, fields
 
 
-# batch-supported launch shapes per grid kind: depth 1 register tiles (rows per
-# lane), f32 LDS temporal tiles of 3 sweeps (16 or 32 rows), bit-packed life
-# tiles of 3, 5 or 7 generations.  The f32 register march of 3 sweeps is a
-# family-wide choice of the batch search (MARCH_F32), not a per-grid candidate.
-MARCH_F32 = "m"
-MARCH_SEG = 32  # rows per march segment (16, 32 and 64 measured on the batch: 32 fastest)
-
-
-def batch_march(shapes, f32_depth):
-    """Per grid: the march's rows per segment (0: not marched) when the f32
-    family runs the register march, every f32 grid over 8 cells wide."""
-    return [MARCH_SEG if f32_depth == MARCH_F32 and kind != "life" and w > 8 else 0 for h, w, kind in shapes]
-
-
-def batch_candidates(kind):
-    from pskel_amd import _lib as L
-
-    def cfg(ty, tz, k):
-        c = L.LaunchCfg()
-        c.variant = L.VARIANT_TUNED
-        c.tile_y, c.tile_z, c.time_tile = ty, tz, k
-        return c
-
-    out = [cfg(ry, 4, 1) for ry in (1, 2, 4, 8)]
-    if kind == "life":
-        out += [cfg(32, 0, k) for k in (3, 5, 7)]
-    else:
-        out += [cfg(cy, 0, 3) for cy in (16, 32)]
-    return out
-
-
-def batch_query(h, w, kind, sweeps):
-    from pskel_amd import tuner as T
-
-    return T.Query((h, w), "u8" if kind == "life" else "f32", "life" if kind == "life" else "jacobi2d", sweeps)
-
-
-def batch_choices(shapes, sweeps):
-    """Per grid: the batch-supported launch shape with the lowest predicted
-    time under the shipped cost model (None without a model)."""
-    from pskel_amd import tuner as T
-
-    model = T.default_model()
-    if model is None:
-        return None
-    out = []
-    for h, w, kind in shapes:
-        cands = batch_candidates(kind)
-        pred = model.predict(T.features(batch_query(h, w, kind, sweeps), cands))
-        out.append(cands[int(np.argmin(pred))])
-    return out
-
-
-def batch_choices_at_depth(shapes, sweeps, life_depth, f32_depth):
-    """Per grid: the model's best batch shape among those of the given depth
-    for its kind (None: the model's pick over every depth; MARCH_F32: depth 3,
-    whose LDS tile stays the shape of a grid the march cannot take)."""
-    from pskel_amd import tuner as T
-
-    model = T.default_model()
-    out = []
-    for h, w, kind in shapes:
-        want = life_depth if kind == "life" else (3 if f32_depth == MARCH_F32 else f32_depth)
-        cands = [c for c in batch_candidates(kind) if want is None or max(1, int(c.time_tile)) == want]
-        pred = model.predict(T.features(batch_query(h, w, kind, sweeps), cands))
-        out.append(cands[int(np.argmin(pred))])
-    return out
-
-
-def batch_depth_search(shapes, timed, sweeps):
-    """The family depths measured on the batch itself.  The cost model prices a
-    grid alone, where a small grid's sweep is one launch of a few microseconds
-    whatever its kernel; inside the batch's persistent launches that floor is
-    shared, and only the work counts.  So each family's depth (life: 1, 3, 5 or
-    7 generations; f32: 1 or 3 sweeps in LDS tiles, or 3 through the register
-    march, MARCH_F32) is timed on the whole batch, with every grid's tile at
-    that depth from the model, next to the model's own picks.
-    Returns (picks, march segments, f32 choice, table of {life_depth,
-    f32_depth, ms_per_step})."""
-    table, best = [], None
-    for ld in (None, 1, 3, 5, 7):
-        for fd in (None, 1, 3, MARCH_F32):
-            if (ld is None) != (fd is None):
-                continue  # the model's picks, or a depth for both families
-            picks = batch_choices_at_depth(shapes, sweeps, ld, fd)
-            rows = [int(c.tile_y) for c in picks]
-            depths = [max(1, int(c.time_tile)) for c in picks]
-            march = batch_march(shapes, fd)
-            ms = timed(rows, depths, march, 2, 1)[0] * 1e3 / 2
-            table.append({"life_depth": ld if ld is not None else "model",
-                          "f32_depth": {None: "model", MARCH_F32: "3 (register march)"}.get(fd, fd),
-                          "ms_per_step": round(ms, 3)})
-            if best is None or ms < best[0]:
-                best = (ms, picks, march, fd)
-    return best[1], best[2], best[3], table
+# The batch's per-grid shapes and family-wide choices (life depth; f32 depth in
+# LDS tiles or through the register march) come from pskel_amd.tuner.batch.
 
 
 def batch_regret_sample(shapes, picks, sweeps, nsample=16, reps=3):
@@ -486,24 +393,24 @@ def batch_regret_sample(shapes, picks, sweeps, nsample=16, reps=3):
 
     from pskel_amd import _lib as L
     from pskel_amd import api
-    from pskel_amd import kernels as K
     from pskel_amd import tuner as T
+    from pskel_amd.tuner import batch as TB
 
     idx = np.linspace(0, len(shapes) - 1, nsample).astype(int)
     rows = []
     for i in idx:
         h, w, kind = shapes[i]
-        q = batch_query(h, w, kind, sweeps)
-        s = K.life() if kind == "life" else K.jacobi2d()
+        q = TB.query(h, w, kind, sweeps)
+        s = TB.stencil_for(kind)
         d = s.desc()
         with api.Grid((h, w), q.storage) as g:
             g.upload_pair(T.field_for(q))
             t = []
-            for c in batch_candidates(kind):
+            for c in TB.candidates(kind):
                 ms = C.c_double()
                 L.check(L.lib().pskel_tuner_measure(g._h, C.byref(d), 24, C.byref(c), reps, C.byref(ms)))
                 t.append(ms.value)
-        keys = [T.cfg_key(c) for c in batch_candidates(kind)]
+        keys = [T.cfg_key(c) for c in TB.candidates(kind)]
         tp = t[keys.index(T.cfg_key(picks[i]))]
         rows.append({"grid": [h, w, kind], "pick": T.cfg_dict(picks[i]), "regret": tp / min(t) - 1.0,
                      "depth1_best_over_best": min(t[:4]) / min(t)})
@@ -513,13 +420,15 @@ def batch_regret_sample(shapes, picks, sweeps, nsample=16, reps=3):
 def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
     """256 uneven grids, `sweeps` sweeps per step: persistent kernels per
     launch, captured in one hipGraph, with every grid's launch shape from the
-    cost model at the family depths measured on the batch (batch_depth_search;
-    the f32 family in LDS tiles or through the register march); beside it the model's own picks, the same batch at depth 1 (register
-    tiles, the model's rows per lane) and the same work launched grid by grid
-    (one runIterativeGPU per grid, the reference's way)."""
+    cost model at the family choices measured on the batch (tuner.batch.search,
+    timed here on the workload's own grids: life depth, f32 in LDS tiles or
+    through the register march); beside it the model's own picks, the same
+    batch at depth 1 (register tiles, the model's rows per lane) and the same
+    work launched grid by grid (one runIterativeGPU per grid, the reference's
+    way)."""
     from pskel_amd import api
     from pskel_amd import kernels as K
-    from pskel_amd import tuner as T
+    from pskel_amd.tuner import batch as TB
 
     shapes, fields = batch_fields()
     grids, stencils = [], []
@@ -551,29 +460,26 @@ def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
 
     search = None
     march, f32_choice = None, None
-    model_picks = batch_choices(shapes, sweeps)
+    model_picks = TB.model_choices(shapes, sweeps)
     if model_picks is None:
         how = "default (depth 1, 4 rows per lane)"
         rows, depths = [4] * len(shapes), [1] * len(shapes)
         rows1 = rows
     else:
         if family_depths is None:
-            how = "cost model tiles at the family depths measured on the batch"
-            picks, march, f32_choice, search = batch_depth_search(shapes, timed, sweeps)
+            how = "cost model tiles at the family choices measured on the batch"
+            chosen = TB.search(shapes, sweeps,
+                               timer=lambda p: timed(p["rows"], p["depths"], p["march"], 2, 1)[0] * 1e3 / 2)
+            f32_choice = chosen["f32_depth"]
+            search = [{"life_depth": r["life_depth"], "f32_depth": r["f32_depth"], "ms_per_step": r["ms_per_run"]}
+                      for r in chosen["table"]]
         else:
             how = f"cost model tiles at life depth {family_depths[0]}, f32 depth {family_depths[1]}"
             f32_choice = family_depths[1]
-            picks = batch_choices_at_depth(shapes, sweeps, *family_depths)
-            march = batch_march(shapes, f32_choice)
-        rows = [int(c.tile_y) for c in picks]
-        depths = [max(1, int(c.time_tile)) for c in picks]
+            chosen = TB.plan(shapes, sweeps, *family_depths)
+        rows, depths, march = chosen["rows"], chosen["depths"], chosen["march"]
         # depth 1 beside it: the model's best register tile per grid
-        rows1 = []
-        model = T.default_model()
-        for h, w, kind in shapes:
-            cands = batch_candidates(kind)[:4]
-            pred = model.predict(T.features(batch_query(h, w, kind, sweeps), cands))
-            rows1.append(int(cands[int(np.argmin(pred))].tile_y))
+        rows1 = TB.plan(shapes, sweeps, 1, 1)["rows"]
 
     try:
         wall, dev, launches, info = timed(rows, depths, march)
@@ -596,8 +502,8 @@ def measure_batch(steps, warmup, sweeps=200, compare=True, family_depths=None):
         "launch_steps": launches,
         "hbm_roofline_frac_per_sweep_bytes": nbytes * sweeps * steps / wall / (HBM_PEAK_GBS * 1e9),
         "depths": {str(k): depths.count(k) for k in sorted(set(depths))},
-        "f32_family": ("register march, 3 sweeps per launch, %d-row segments" % MARCH_SEG
-                       if f32_choice == MARCH_F32 else "LDS temporal tiles" if 3 in depths else "register tiles"),
+        "f32_family": ("register march, 3 sweeps per launch, %d-row segments" % TB.MARCH_SEG
+                       if f32_choice == TB.MARCH else "LDS temporal tiles" if 3 in depths else "register tiles"),
         "march_grids": sum(1 for m in (march or []) if m),
         "items": info["items"],
         "persistent_blocks": info["blocks"],
@@ -832,10 +738,12 @@ def main(argv=None):
         return copy_calibration(args.copy_mib, max(args.steps, 1))
     if args.batch_only:
         # the batch alone (a profiling target): the depth search, or the given family depths
+        from pskel_amd.tuner import batch as TB
+
         depths = None
         if args.batch_depths:
             life_d, f32_d = args.batch_depths.split(",")
-            depths = (int(life_d), f32_d if f32_d == MARCH_F32 else int(f32_d))
+            depths = (int(life_d), f32_d if f32_d == TB.MARCH else int(f32_d))
         print(json.dumps(measure_batch(max(args.steps, 1), args.warmup, compare=False, family_depths=depths)))
         return 0
 

@@ -102,6 +102,28 @@ def test_batch_register_march(gpu, use_graph):
             g.close()
 
 
+def test_batch_search(gpu):
+    """The batch-level family search (pskel_amd.tuner.batch.search) on scratch
+    grids: every family choice is timed, the fastest plan is returned, and a
+    batch run with it is bit-exact."""
+    from pskel_amd.tuner import batch as TB
+
+    shapes = [(h, w, "life" if kind == "life" else "jacobi") for h, w, kind in MARCH_SHAPES]
+    res = TB.search(shapes, 12, reps=1)
+    assert res is not None and len(res["table"]) == len(TB.FAMILY_CHOICES)
+    assert abs(res["ms_per_run"] - min(r["ms_per_run"] for r in res["table"])) < 1e-3  # the table rounds to 1 us
+    assert len(res["rows"]) == len(res["depths"]) == len(res["march"]) == len(shapes)
+    grids, stencils, fields = _make(shapes, seed=13)
+    try:
+        with api.Batch(grids, stencils, res["rows"], res["depths"], res["march"]) as b:
+            b.run(12, use_graph=True)
+        for g, s, f in zip(grids, stencils, fields):
+            assert np.array_equal(g.download(), oracle.run_2d(s, f, 12)), g.shape
+    finally:
+        for g in grids:
+            g.close()
+
+
 def test_batch_march_validation(gpu):
     grids, stencils, _ = _make([(20, 8, "jacobi"), (64, 64, "jacobi")], seed=12)
     try:

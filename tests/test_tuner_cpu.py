@@ -142,7 +142,7 @@ def test_features_shape_and_determinism():
     assert abs(f1[0, 8] - np.log2(50)) < 1e-5
 
 
-def test_heuristic_picks_a_candidate():
+def test_heuristic_picks_a_candidate_on_the_workloads():
     for q in (_q((2048, 2048), "f32", "jacobi2d"), _q((512, 512, 512), "f32", "jacobi3d"),
               _q((64, 64), "u8", "life", 3)):
         h = T.heuristic_choice(q)
@@ -203,3 +203,17 @@ def test_saved_timing_rows_round_trip():
     js[0] = dict(js[0], cfg=[99, 9, 9, 9, 9, 9, 9])
     back, missing = T.rows_from_json(js)
     assert len(back) == len(rows) - 1 and len(missing) == 1
+
+
+def test_batch_family_choices():
+    """The batch search's family choices and march segments (no device)."""
+    from pskel_amd.tuner import batch as TB
+
+    assert TB.FAMILY_CHOICES[0] == (None, None) and len(TB.FAMILY_CHOICES) == 13
+    shapes = [(64, 64, "jacobi"), (300, 8, "jacobi"), (70, 1030, "life"), (129, 513, "diffusion9")]
+    assert TB.march_segments(shapes, TB.MARCH) == [TB.MARCH_SEG, 0, 0, TB.MARCH_SEG]
+    assert TB.march_segments(shapes, 3) == [0, 0, 0, 0]
+    for kind in ("jacobi", "diffusion9", "life"):
+        depths = sorted({max(1, int(c.time_tile)) for c in TB.candidates(kind)})
+        assert depths == ([1, 3, 5, 7] if kind == "life" else [1, 3])
+        assert TB.query(100, 200, kind, 50).storage == ("u8" if kind == "life" else "f32")
