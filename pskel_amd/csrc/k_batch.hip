@@ -388,9 +388,19 @@ pskel_status_t pskel_batch_create(pskel_grid_t* const* grids, const pskel_stenci
     hipError_t e = hipSetDevice(b->device);
     int cus = 256;
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
-    // persistent: a few resident blocks per CU, never more blocks than items
+    // persistent: at most as many blocks as the device holds at once (the
+    // family kernel's occupancy: registers and LDS), never more than items, so
+    // no block waits for a slot with a full share of items
     b->nblocks = std::max(1, std::min(b->nitems, cus * 4));
-    for (int f = 0; f < BF_COUNT; ++f) b->fam_blocks[f] = std::max(1, std::min(b->fam_items[f], cus * 4));
+    const void* fam_kernel[BF_COUNT] = {reinterpret_cast<const void*>(&k_batch<BF_REG>),
+                                        reinterpret_cast<const void*>(&k_batch<BF_TTILE>),
+                                        reinterpret_cast<const void*>(&k_batch<BF_BITS>),
+                                        reinterpret_cast<const void*>(&k_batch<BF_MARCH>)};
+    for (int f = 0; f < BF_COUNT; ++f) {
+        int per_cu = 4;
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fam_kernel[f], kWave * kBatchWaves, 0);
+        b->fam_blocks[f] = std::max(1, std::min(b->fam_items[f], cus * std::max(1, per_cu)));
+    }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
